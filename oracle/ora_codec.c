@@ -1,0 +1,465 @@
+/* Oracle (test infrastructure only): encode/decode orchestration and the
+ * flat C API the tests and the CPU-baseline leg load through ctypes.
+ * Restates encode/mod.rs:84-101, encode/context.rs:27-70,
+ * encode/backend/native/backend.rs:52-242, :436-517, layout.rs:37-173,
+ * decode/mod.rs:56-129, decode/t1.rs:32-73 and decode/reconstruct.rs. */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ora_t2.h"
+
+static int band_gain(int b) { return b == ORA_LL ? 0 : (b == ORA_HH ? 2 : 1); }
+
+/* subband rectangle of (res, band) in the Mallat plane (layout.rs:37-97) */
+static void band_rect(size_t w, size_t h, int levels, int res, int band, int *x0, int *y0, int *x1, int *y1) {
+    size_t rw[33], rh[33];
+    ora_resolutions(w, h, levels, rw, rh);
+    if (res == 0) {
+        *x0 = 0; *y0 = 0; *x1 = (int)rw[0]; *y1 = (int)rh[0];
+    } else if (band == ORA_HL) {
+        *x0 = (int)rw[res - 1]; *y0 = 0; *x1 = (int)rw[res]; *y1 = (int)rh[res - 1];
+    } else if (band == ORA_LH) {
+        *x0 = 0; *y0 = (int)rh[res - 1]; *x1 = (int)rw[res - 1]; *y1 = (int)rh[res];
+    } else {
+        *x0 = (int)rw[res - 1]; *y0 = (int)rh[res - 1]; *x1 = (int)rw[res]; *y1 = (int)rh[res];
+    }
+}
+
+/* context.rs:39-70: integer BT.601 YCbCr -> RGB for the hidden colour spaces */
+static void yuv_to_rgb(const int32_t *const *in, size_t n, int32_t **out) {
+    for (size_t i = 0; i < n; i++) {
+        int32_t y = in[0][i], d = in[1][i] - 128, e = in[2][i] - 128;
+        int32_t r = y + ((91881 * e) >> 16);
+        int32_t g = y - ((22554 * d + 46802 * e) >> 16);
+        int32_t b = y + ((116130 * d) >> 16);
+        out[0][i] = r < 0 ? 0 : (r > 255 ? 255 : r);
+        out[1][i] = g < 0 ? 0 : (g > 255 ? 255 : g);
+        out[2][i] = b < 0 ? 0 : (b > 255 ? 255 : b);
+    }
+}
+
+/* Quantised Mallat coefficient plane of one component
+ * (backend.rs:52-84 lossy, :118-159 lossless). */
+static void component_coefficients(const ora_plan *p, const int32_t *const *src, int c, int32_t *out) {
+    size_t n = (size_t)p->width * p->height;
+    if (p->lossless) {
+        if (p->use_mct) ora_forward_rct(src[0], src[1], src[2], n, c, out);
+        else ora_level_shift_i32(src[c], n, out);
+        ora_fwd53(out, p->width, p->height, p->levels);
+    } else {
+        float *f = malloc(sizeof(float) * n);
+        if (p->use_mct) ora_forward_ict(src[0], src[1], src[2], n, c, f);
+        else ora_level_shift_f32(src[c], n, f);
+        ora_fwd97(f, p->width, p->height, p->levels);
+        ora_quantize_plane(f, p->width, p->height, p->levels, p, out);
+        free(f);
+    }
+}
+
+/* Mb of a subband (t1.rs:19-22 lossless with backend.rs:179-189 effective
+ * precision; backend.rs:190-205 lossy: G - 1 + exponent, u8 saturating). */
+static int subband_mb(const ora_plan *p, int c, int res, int band) {
+    if (p->lossless) {
+        int prec = (p->use_mct && c > 0) ? 9 : 8;
+        int v = (p->guard - 1) + prec + band_gain(band);
+        return v > 255 ? 255 : v;
+    }
+    for (int i = 0; i < p->nbands; i++)
+        if (p->sq[i].resolution == res && p->sq[i].band == band) {
+            int v = (p->guard - 1) + p->sq[i].exponent;
+            return v > 255 ? 255 : v;
+        }
+    return (p->guard - 1) + 8 + band_gain(band);
+}
+
+static int count_blocks(int x0, int x1, int cb) { return x1 > x0 ? (x1 - x0 + cb - 1) / cb : 0; }
+
+int ora_encode(const int32_t *const *planes, uint32_t w, uint32_t h, int ncomp, int colorspace,
+               int quality, int format, ora_encoding *enc) {
+    memset(enc, 0, sizeof(*enc));
+    ora_plan *p = &enc->plan;
+    int rc = ora_build_plan(planes, w, h, ncomp, colorspace, quality, format, p, enc->err);
+    if (rc) return rc;
+    size_t n = (size_t)w * h;
+    const int32_t *src[3] = {planes[0], ncomp > 1 ? planes[1] : NULL, ncomp > 2 ? planes[2] : NULL};
+    int32_t *conv[3] = {NULL, NULL, NULL};
+    if (colorspace == ORA_YUV || colorspace == ORA_YCBCR) {
+        for (int c = 0; c < 3; c++) conv[c] = malloc(sizeof(int32_t) * n);
+        yuv_to_rgb(planes, n, conv);
+        for (int c = 0; c < 3; c++) src[c] = conv[c];
+    }
+    int32_t **coef = malloc(sizeof(int32_t *) * (size_t)ncomp);
+    for (int c = 0; c < ncomp; c++) {
+        coef[c] = malloc(sizeof(int32_t) * n);
+        component_coefficients(p, src, c, coef[c]);
+    }
+    /* blocks in codestream order: resolution, component, band, raster */
+    int total = 0;
+    for (int r = 0; r <= p->levels; r++)
+        for (int c = 0; c < ncomp; c++)
+            for (int b = (r ? ORA_HL : ORA_LL); b <= (r ? ORA_HH : ORA_LL); b++) {
+                int x0, y0, x1, y1;
+                band_rect(w, h, p->levels, r, b, &x0, &y0, &x1, &y1);
+                total += count_blocks(x0, x1, p->cbw) * count_blocks(y0, y1, p->cbh);
+            }
+    enc->nblocks = total;
+    enc->blocks = calloc((size_t)(total ? total : 1), sizeof(ora_block));
+    int k = 0;
+    for (int r = 0; r <= p->levels; r++)
+        for (int c = 0; c < ncomp; c++)
+            for (int b = (r ? ORA_HL : ORA_LL); b <= (r ? ORA_HH : ORA_LL); b++) {
+                int x0, y0, x1, y1;
+                band_rect(w, h, p->levels, r, b, &x0, &y0, &x1, &y1);
+                for (int by = y0; by < y1; by += p->cbh)
+                    for (int bx = x0; bx < x1; bx += p->cbw) {
+                        ora_block *blk = &enc->blocks[k++];
+                        blk->comp = c;
+                        blk->res = r;
+                        blk->band = b;
+                        blk->x0 = bx;
+                        blk->y0 = by;
+                        blk->x1 = bx + p->cbw < x1 ? bx + p->cbw : x1;
+                        blk->y1 = by + p->cbh < y1 ? by + p->cbh : y1;
+                    }
+            }
+    /* Tier-1, parallel over code-blocks like the reference's rayon loops
+     * (t1.rs:191-196, :287-290) */
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < total; i++) {
+        ora_block *blk = &enc->blocks[i];
+        int bw = blk->x1 - blk->x0, bh = blk->y1 - blk->y0;
+        int32_t *cb = malloc(sizeof(int32_t) * (size_t)(bw * bh));
+        for (int y = 0; y < bh; y++)
+            memcpy(cb + y * bw, coef[blk->comp] + (size_t)(blk->y0 + y) * w + blk->x0, sizeof(int32_t) * (size_t)bw);
+        ora_t1_encode_block(cb, bw, bh, blk->band, subband_mb(p, blk->comp, blk->res, blk->band), blk);
+        free(cb);
+    }
+    for (int c = 0; c < ncomp; c++) free(coef[c]);
+    free(coef);
+    for (int c = 0; c < 3; c++) free(conv[c]);
+    ora_pcrd_apply(enc, p);
+    /* Tier-2: one packet per (resolution, component) (t2.rs:409-470) */
+    obuf pay = {0};
+    int bstart = 0;
+    for (int r = 0; r <= p->levels; r++)
+        for (int c = 0; c < ncomp; c++) {
+            ora_band_ref refs[3];
+            int nb = 0;
+            for (int b = (r ? ORA_HL : ORA_LL); b <= (r ? ORA_HH : ORA_LL); b++) {
+                int x0, y0, x1, y1;
+                band_rect(w, h, p->levels, r, b, &x0, &y0, &x1, &y1);
+                int cx = count_blocks(x0, x1, p->cbw), cy = count_blocks(y0, y1, p->cbh);
+                refs[nb].nblocks = cx * cy;
+                refs[nb].cx = cx;
+                refs[nb].cy = cy;
+                refs[nb].blocks = malloc(sizeof(ora_block *) * (size_t)(cx * cy ? cx * cy : 1));
+                for (int i = 0; i < cx * cy; i++) refs[nb].blocks[i] = &enc->blocks[bstart + i];
+                bstart += cx * cy;
+                nb++;
+            }
+            ora_packet_header(refs, nb, &pay);
+            for (int i = 0; i < nb; i++) {
+                for (int j = 0; j < refs[i].nblocks; j++) {
+                    ora_block *blk = refs[i].blocks[j];
+                    if (blk->npasses) obuf_put(&pay, blk->bytes, (size_t)blk->body_len);
+                }
+                free(refs[i].blocks);
+            }
+        }
+    obuf cs = {0};
+    ora_write_codestream(p, pay.data, pay.len, &cs);
+    free(pay.data);
+    enc->codestream = cs.data;
+    enc->codestream_len = cs.len;
+    if (format == 0) {
+        obuf jp = {0};
+        ora_wrap_jp2(p, cs.data, cs.len, &jp);
+        enc->file = jp.data;
+        enc->file_len = jp.len;
+    } else {
+        enc->file = malloc(cs.len);
+        memcpy(enc->file, cs.data, cs.len);
+        enc->file_len = cs.len;
+    }
+    return ORA_OK;
+}
+
+void ora_encoding_free(ora_encoding *enc) {
+    for (int i = 0; i < enc->nblocks; i++) free(enc->blocks[i].bytes);
+    free(enc->blocks);
+    free(enc->codestream);
+    free(enc->file);
+    memset(enc, 0, sizeof(*enc));
+}
+
+/* decode/mod.rs:56-129 + decode/t1.rs:32-73 + decode/reconstruct.rs:24-302 */
+int ora_decode(const uint8_t *bytes, size_t len, uint32_t *ow, uint32_t *oh, int *onc, int *ocs,
+               int32_t **oplanes, char *err) {
+    ora_jp2_header jh;
+    const uint8_t *cs;
+    size_t cslen;
+    memset(&jh, 0, sizeof(jh));
+    int rc = ora_parse_jp2(bytes, len, &jh, &cs, &cslen, err);
+    if (rc) return rc;
+    ora_cs_header hd;
+    rc = ora_parse_codestream(cs, cslen, &hd, err);
+    if (rc) return rc;
+    if (jh.width != hd.width || jh.height != hd.height) {
+        snprintf(err, 256, "JP2 ihdr dimensions do not match SIZ dimensions");
+        return ORA_DECODE_FAILED;
+    }
+    if (jh.bpc != 8) {
+        snprintf(err, 256, "unsupported JP2 bit depth: decoder currently supports 8-bit components, found %d bits", jh.bpc);
+        return ORA_UNSUPPORTED;
+    }
+    if (jh.ncomp != hd.ncomp) {
+        snprintf(err, 256, "JP2 ihdr component count does not match SIZ component count");
+        return ORA_DECODE_FAILED;
+    }
+    if (jh.colorspace == ORA_GRAY && jh.ncomp != 1) {
+        snprintf(err, 256, "unsupported JP2 component count");
+        return ORA_UNSUPPORTED;
+    }
+    if (jh.colorspace == ORA_SRGB && jh.ncomp != 3) {
+        snprintf(err, 256, "unsupported JP2 component count");
+        return ORA_UNSUPPORTED;
+    }
+    uint32_t w = hd.width, h = hd.height;
+    int L = hd.levels, nc = hd.ncomp;
+    /* band states in component-major order (decode/t2.rs:266-340) */
+    int nbands = nc * (1 + 3 * L);
+    ora_dec_band *bands = calloc((size_t)nbands, sizeof(ora_dec_band));
+    int nblocks = 0, bi = 0;
+    for (int c = 0; c < nc; c++)
+        for (int r = 0; r <= L; r++)
+            for (int b = (r ? ORA_HL : ORA_LL); b <= (r ? ORA_HH : ORA_LL); b++) {
+                int x0, y0, x1, y1;
+                band_rect(w, h, L, r, b, &x0, &y0, &x1, &y1);
+                int cx = count_blocks(x0, x1, hd.cbw), cy = count_blocks(y0, y1, hd.cbh);
+                bands[bi] = (ora_dec_band){c, r, b, nblocks, cx * cy, cx, cy};
+                nblocks += cx * cy;
+                bi++;
+            }
+    ora_dec_block *blocks = calloc((size_t)(nblocks ? nblocks : 1), sizeof(ora_dec_block));
+    for (int i = 0; i < nbands; i++) {
+        int x0, y0, x1, y1;
+        band_rect(w, h, L, bands[i].res, bands[i].band, &x0, &y0, &x1, &y1);
+        int k = bands[i].first;
+        for (int by = y0; by < y1; by += hd.cbh)
+            for (int bx = x0; bx < x1; bx += hd.cbw) {
+                ora_dec_block *blk = &blocks[k++];
+                blk->comp = bands[i].comp;
+                blk->res = bands[i].res;
+                blk->band = bands[i].band;
+                blk->band_index = i;
+                blk->x0 = bx;
+                blk->y0 = by;
+                blk->x1 = bx + hd.cbw < x1 ? bx + hd.cbw : x1;
+                blk->y1 = by + hd.cbh < y1 ? by + hd.cbh : y1;
+                blk->zbp = -1;
+                blk->numlenbits = 3;
+            }
+    }
+    rc = ora_parse_packets(&hd, blocks, nblocks, bands, nbands, err);
+    if (rc) {
+        free(bands);
+        free(blocks);
+        return rc;
+    }
+    size_t n = (size_t)w * h;
+    int32_t *coef = calloc(n * (size_t)nc, sizeof(int32_t));
+    /* the reference decodes code-blocks serially (decode/t1.rs:50) */
+    for (int i = 0; i < nblocks && !rc; i++) {
+        ora_dec_block *blk = &blocks[i];
+        if (!blk->included) continue;
+        int step_i = blk->band_index % hd.nsteps;
+        int mb = (hd.guard > 0 ? hd.guard - 1 : 0) + hd.exp[step_i];
+        if (mb > 255) mb &= 255;
+        int bw = blk->x1 - blk->x0, bh = blk->y1 - blk->y0;
+        int32_t *out = malloc(sizeof(int32_t) * (size_t)(bw * bh));
+        rc = ora_t1_decode_block(bw, bh, blk->band, mb, blk->zbp, blk->passes, blk->data, blk->len, out, err);
+        if (!rc)
+            for (int y = 0; y < bh; y++)
+                memcpy(coef + (size_t)blk->comp * n + (size_t)(blk->y0 + y) * w + blk->x0, out + y * bw,
+                       sizeof(int32_t) * (size_t)bw);
+        free(out);
+    }
+    free(blocks);
+    if (rc) {
+        free(bands);
+        free(coef);
+        return rc;
+    }
+    int32_t *planes = malloc(sizeof(int32_t) * n * (size_t)nc);
+    if (hd.transform == 1) {
+        for (int c = 0; c < nc; c++) ora_inv53(coef + (size_t)c * n, w, h, L);
+        if (nc == 3 && hd.mct) ora_inverse_rct(coef, coef + n, coef + 2 * n, n);
+        for (int c = 0; c < nc; c++) ora_finalize_i32(coef + (size_t)c * n, n, planes + (size_t)c * n);
+    } else {
+        float *f = malloc(sizeof(float) * n * (size_t)nc);
+        for (int c = 0; c < nc; c++) {
+            float *fc = f + (size_t)c * n;
+            for (int b = 0; b < 1 + 3 * L; b++) {
+                int res = b == 0 ? 0 : (b - 1) / 3 + 1;
+                int band = b == 0 ? ORA_LL : (b - 1) % 3 + 1;
+                int x0, y0, x1, y1;
+                band_rect(w, h, L, res, band, &x0, &y0, &x1, &y1);
+                float step = ora_quant_step_f32((uint32_t)hd.prec[c], band, hd.exp[b], hd.mant[b]);
+                for (int y = y0; y < y1; y++)
+                    for (int x = x0; x < x1; x++)
+                        fc[(size_t)y * w + x] = ora_dequantize(coef[(size_t)c * n + (size_t)y * w + x], step);
+            }
+            ora_inv97(fc, w, h, L);
+        }
+        if (nc == 3 && hd.mct) ora_inverse_ict(f, f + n, f + 2 * n, n);
+        for (int c = 0; c < nc; c++) ora_finalize_f32(f + (size_t)c * n, n, planes + (size_t)c * n);
+        free(f);
+    }
+    free(coef);
+    free(bands);
+    *ow = w;
+    *oh = h;
+    *onc = nc;
+    *ocs = jh.colorspace;
+    *oplanes = planes;
+    return ORA_OK;
+}
+
+/* ---- flat API for ctypes ---------------------------------------------------- */
+static void u8_to_planes(const uint8_t *pix, size_t n, int ch, int32_t **planes) {
+    for (int c = 0; c < ch; c++) {
+        planes[c] = malloc(sizeof(int32_t) * n);
+        for (size_t i = 0; i < n; i++) planes[c][i] = pix[i * (size_t)ch + (size_t)c];
+    }
+}
+
+void ora_free(void *p) { free(p); }
+
+/* Encode interleaved u8 pixels (gray: 1 channel, sRGB: 3) like
+ * Image::from_gray_bytes / from_rgb_bytes + encode (model.rs:87-140). */
+ora_encoding *ora_encode_u8(const uint8_t *pix, uint32_t w, uint32_t h, int channels, int quality, int format, int *status) {
+    ora_encoding *e = calloc(1, sizeof(*e));
+    int32_t *planes[3] = {0};
+    u8_to_planes(pix, (size_t)w * h, channels, planes);
+    *status = ora_encode((const int32_t *const *)planes, w, h, channels, channels == 1 ? ORA_GRAY : ORA_SRGB,
+                         quality, format, e);
+    for (int c = 0; c < channels; c++) free(planes[c]);
+    return e;
+}
+ora_encoding *ora_encode_planes(const int32_t *planes_flat, uint32_t w, uint32_t h, int ncomp, int colorspace,
+                                int quality, int format, int *status) {
+    ora_encoding *e = calloc(1, sizeof(*e));
+    const int32_t *planes[3] = {0};
+    for (int c = 0; c < ncomp && c < 3; c++) planes[c] = planes_flat + (size_t)c * w * h;
+    *status = ora_encode(planes, w, h, ncomp, colorspace, quality, format, e);
+    return e;
+}
+void ora_enc_release(ora_encoding *e) {
+    if (!e) return;
+    ora_encoding_free(e);
+    free(e);
+}
+const char *ora_enc_error(const ora_encoding *e) { return e->err; }
+size_t ora_enc_file(const ora_encoding *e, const uint8_t **data) {
+    *data = e->file;
+    return e->file_len;
+}
+size_t ora_enc_codestream(const ora_encoding *e, const uint8_t **data) {
+    *data = e->codestream;
+    return e->codestream_len;
+}
+int ora_enc_nblocks(const ora_encoding *e) { return e->nblocks; }
+double ora_enc_lambda(const ora_encoding *e, int c) { return e->lambda[c]; }
+void ora_enc_plan(const ora_encoding *e, int32_t *out) {
+    const ora_plan *p = &e->plan;
+    out[0] = p->levels;
+    out[1] = p->cbw;
+    out[2] = p->nbands;
+    out[3] = p->use_mct;
+    out[4] = p->lossless;
+    for (int i = 0; i < p->nbands; i++) {
+        out[5 + 2 * i] = p->sq[i].exponent;
+        out[6 + 2 * i] = p->sq[i].mantissa;
+    }
+}
+/* info[16] = comp res band x0 y0 x1 y1 mbp zbp cls npasses full_npasses nbytes body_len max nnz */
+void ora_enc_block_info(const ora_encoding *e, int i, int32_t *info) {
+    const ora_block *b = &e->blocks[i];
+    int32_t v[16] = {b->comp, b->res, b->band, b->x0, b->y0, b->x1, b->y1, b->mbp, b->zbp, b->cls,
+                     b->npasses, b->full_npasses, b->nbytes, b->body_len, (int32_t)b->max_magnitude, b->nnz};
+    memcpy(info, v, sizeof(v));
+}
+/* per full pass: kind bitplane newsig sigbefore length */
+void ora_enc_block_passes(const ora_encoding *e, int i, int32_t *out) {
+    const ora_block *b = &e->blocks[i];
+    for (int k = 0; k < b->full_npasses; k++) {
+        out[5 * k + 0] = b->passes[k].kind;
+        out[5 * k + 1] = b->passes[k].bitplane;
+        out[5 * k + 2] = b->passes[k].newly_significant;
+        out[5 * k + 3] = b->passes[k].significant_before;
+        out[5 * k + 4] = b->full_lengths[k];
+    }
+}
+void ora_enc_block_bytes(const ora_encoding *e, int i, uint8_t *out) {
+    const ora_block *b = &e->blocks[i];
+    if (b->nbytes) memcpy(out, b->bytes, (size_t)b->nbytes);
+}
+
+/* Quantised coefficient planes (Mallat layout) as Tier-1 sees them. */
+int ora_coefficients_u8(const uint8_t *pix, uint32_t w, uint32_t h, int channels, int quality, int32_t *out) {
+    int32_t *planes[3] = {0};
+    size_t n = (size_t)w * h;
+    u8_to_planes(pix, n, channels, planes);
+    ora_plan p;
+    char err[256];
+    int rc = ora_build_plan((const int32_t *const *)planes, w, h, channels, channels == 1 ? ORA_GRAY : ORA_SRGB,
+                            quality, 0, &p, err);
+    if (!rc)
+        for (int c = 0; c < channels; c++) component_coefficients(&p, (const int32_t *const *)planes, c, out + (size_t)c * n);
+    for (int c = 0; c < channels; c++) free(planes[c]);
+    return rc;
+}
+
+/* Decode to interleaved u8 (caller buffer w*h*ncomp, sized via inspect). */
+int ora_decode_u8(const uint8_t *bytes, size_t len, uint32_t *w, uint32_t *h, int *nc, uint8_t *out, size_t cap,
+                  char *err) {
+    int32_t *planes = NULL;
+    int cs;
+    int rc = ora_decode(bytes, len, w, h, nc, &cs, &planes, err);
+    if (rc) return rc;
+    size_t n = (size_t)*w * *h;
+    if (n * (size_t)*nc > cap) {
+        free(planes);
+        snprintf(err, 256, "output buffer too small");
+        return ORA_IO;
+    }
+    for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < *nc; c++) out[i * (size_t)*nc + (size_t)c] = (uint8_t)planes[(size_t)c * n + i];
+    free(planes);
+    return ORA_OK;
+}
+
+/* Tier-1 block coder entry points for the block-level parity tests. */
+int ora_t1_encode_flat(const int32_t *coef, int w, int h, int band, int mb, int32_t *info, int32_t *passes,
+                       uint8_t *bytes, int cap) {
+    ora_block b;
+    memset(&b, 0, sizeof(b));
+    ora_t1_encode_block(coef, w, h, band, mb, &b);
+    info[0] = b.mbp;
+    info[1] = b.zbp;
+    info[2] = b.cls;
+    info[3] = b.npasses;
+    info[4] = b.nbytes;
+    for (int k = 0; k < b.npasses; k++) {
+        passes[5 * k + 0] = b.passes[k].kind;
+        passes[5 * k + 1] = b.passes[k].bitplane;
+        passes[5 * k + 2] = b.passes[k].newly_significant;
+        passes[5 * k + 3] = b.passes[k].significant_before;
+        passes[5 * k + 4] = b.passes[k].length;
+    }
+    int rc = b.nbytes <= cap ? ORA_OK : ORA_ENCODE_FAILED;
+    if (!rc && b.nbytes) memcpy(bytes, b.bytes, (size_t)b.nbytes);
+    free(b.bytes);
+    return rc;
+}
