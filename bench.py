@@ -1,0 +1,175 @@
+"""Benchmark: encode + decode Mpixel/s of one 2550x3300 8-bit grayscale
+document page at quality 85 (9/7, 5 levels, 64x64 code-blocks, JP2), with
+the page resident in HBM and the decoded pixels written back to HBM.
+
+One step = encode the page (device pixels -> JP2 bytes on the host, Tier-2
+on the host) + decode those bytes (host Tier-2 parse -> device pixels).
+Prints one JSON line; see --help.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quality", type=int, default=85)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--no-check", action="store_true", help="skip the post-run oracle check")
+    return ap.parse_args()
+
+
+def level_areas(w, h, levels):
+    areas = []
+    for _ in range(levels):
+        areas.append(w * h)
+        w, h = (w + 1) // 2, (h + 1) // 2
+    return areas
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU: every box here has a single MI355X; pages shard across GPUs "
+                 "without exchange (see DESIGN.md), so run it with --gpus 1")
+    import numpy as np
+    import torch
+
+    import jp2lam_amd as j
+    from jp2lam_amd import _lib, synth
+
+    lib = _lib.load()
+    if not torch.cuda.is_available() or not lib.jp2g_device_available():
+        sys.exit("bench.py needs a HIP device")
+    W, H, Q = 2550, 3300, args.quality
+    px = synth.document_page(W, H, seed=2)
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(px).to(dev)
+    d_out = torch.empty_like(d_in)
+    torch.cuda.synchronize()
+
+    tim = (np.zeros(4, np.float64), np.zeros(4, np.float64))
+
+    def step():
+        enc = j.encode_many([((d_in.data_ptr(), H, W, 1), Q)], device=True)[0]
+        if isinstance(enc, Exception):
+            raise enc
+        t = np.zeros(4, np.float64)
+        lib.jp2g_last_timings(t.ctypes.data)
+        res = j.decode_many([enc], device_outputs=[(d_out.data_ptr(), d_out.numel())])[0]
+        if isinstance(res, Exception):
+            raise res
+        t2 = np.zeros(4, np.float64)
+        lib.jp2g_last_timings(t2.ctypes.data)
+        return enc, t, t2
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    enc_t = np.zeros(4)
+    dec_t = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        enc, te, td = step()
+        enc_t += te
+        dec_t += td
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1000.0 / args.steps
+    enc_t /= args.steps
+    dec_t /= args.steps
+    mpx = W * H / 1e6 / (ms / 1000.0)
+
+    # ---- roofline of the dominant kernel (HIP events on the engine stream) ----
+    P = W * H
+    areas = level_areas(W, H, 5)
+    stages = {
+        # name: (avg ms, algorithmic bytes per launch, bound)
+        "encode_dwt (level shift + 5 levels)": (enc_t[0], P * (1 + 4) + 8 * sum(areas)),
+        "t1_encode_kernel": (enc_t[1], 4 * P + len(enc)),
+        "pcrd+pack": (enc_t[2], 0),
+        "t1_decode_kernel": (dec_t[1], len(enc) + 4 * P),
+        "decode_idwt (5 levels + finalize)": (dec_t[0], 8 * sum(areas) + P * (4 + 1)),
+    }
+    dom = max(stages, key=lambda k: stages[k][0])
+    dms, dbytes = stages[dom]
+    achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "avg_ms": round(dms, 4)}
+    stage_ms = {k: round(v[0], 4) for k, v in stages.items()}
+    stage_ms["encode_host_tier2"] = round(enc_t[3], 4)
+
+    # ---- parity check of the measured output (outside the timed region) ----
+    check = None
+    if not args.no_check:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle_ctypes import Oracle
+
+        ora = Oracle()
+        ref = ora.encode(px, Q)
+        out = d_out.cpu().numpy()
+        check = bool(enc == ref and np.array_equal(out, ora.decode(ref)))
+
+    # ---- CPU baseline: the oracle (C restatement, OpenMP) on the same page ----
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle_ctypes import Oracle
+
+        ora = Oracle()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ora.encode(px, Q)
+        runs = []
+        for _ in range(args.cpu_runs):
+            a = time.perf_counter()
+            d = ora.encode(px, Q)
+            ora.decode(d)
+            runs.append(time.perf_counter() - a)
+        med = statistics.median(runs)
+        cpu = {"value": round(P / 1e6 / med, 3), "unit": "Mpixel/s", "cores": threads, "kind": "port",
+               "sample": f"full 2550x3300 q{Q} page, encode+decode, median of {args.cpu_runs} runs after 1 warm-up; "
+                         f"oracle C restatement, T1 encode over OpenMP threads, T1 decode serial like the reference, "
+                         f"contrast-mask dead work skipped (lean)"}
+
+    line = {
+        "metric": "encode+decode Mpixel/s on 1 GPU vs plain GPU & CPU; p50/p99 page latency",
+        "value": round(mpx, 3),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded document page: background 245+N(0,3), 12% dark glyph rectangles)",
+        "config": {"workload": "plain per-image GPU path: 2550x3300 gray document page, q85 9/7 5 levels, "
+                               "64x64 code-blocks, JP2; encode then decode, pixels resident in HBM",
+                   "width": W, "height": H, "quality": Q, "codestream_bytes": len(enc)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stage_ms": stage_ms,
+        "parity_vs_oracle": check,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

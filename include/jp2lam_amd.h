@@ -1,0 +1,182 @@
+/*
+ * jp2lam_amd: C ABI of the MI355X JPEG 2000 Part 1 codec core.
+ *
+ * The entry points mirror the public API of the jp2lam crate so that a
+ * Rust `build.rs` + `extern "C"` shim can bind them one for one (see
+ * INTEGRATION.md).  Reference locations are given as crate-relative paths.
+ *
+ * Conventions
+ *  - Every function returning jp2lam_status sets a thread-local message
+ *    readable with jp2lam_last_error() when it fails.
+ *  - Status codes map 1:1 onto Jp2LamError's variants (src/error.rs:6-13).
+ *  - Buffers returned through `uint8_t** out` and jp2lam_image are owned by
+ *    the library; release them with jp2lam_free / jp2lam_image_free.
+ *  - Images hold planar i32 samples like the crate's Image (src/model.rs:64-81).
+ */
+#ifndef JP2LAM_AMD_H
+#define JP2LAM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Jp2LamError variants (src/error.rs:6-13). */
+typedef enum {
+    JP2LAM_OK = 0,
+    JP2LAM_INVALID_INPUT = 1,
+    JP2LAM_ENCODE_FAILED = 2,
+    JP2LAM_DECODE_FAILED = 3,
+    JP2LAM_UNSUPPORTED_FEATURE = 4,
+    JP2LAM_IO = 5
+} jp2lam_status;
+
+/* ColorSpace (src/model.rs:1-12). */
+typedef enum {
+    JP2LAM_GRAY = 0,
+    JP2LAM_SRGB = 1,
+    JP2LAM_RGB = 2,
+    JP2LAM_YUV = 3,
+    JP2LAM_YCBCR = 4
+} jp2lam_colorspace;
+
+/* OutputFormat (src/model.rs:29-33). */
+typedef enum { JP2LAM_FORMAT_JP2 = 0, JP2LAM_FORMAT_J2K = 1 } jp2lam_format;
+
+/* EncodeOptions{quality, format} (src/model.rs:155-177); quality >= 100 is
+ * lossless (src/plan/mod.rs:121). */
+typedef struct {
+    uint8_t quality;
+    uint8_t format;
+} jp2lam_encode_options;
+
+/* Component (src/model.rs:64-73), borrowed. */
+typedef struct {
+    const int32_t* data;
+    uint32_t width, height, precision;
+    uint32_t is_signed;
+    uint32_t dx, dy;
+} jp2lam_component;
+
+/* Image (src/model.rs:75-81), borrowed view. */
+typedef struct {
+    uint32_t width, height;
+    uint32_t colorspace;
+    uint32_t num_components;
+    const jp2lam_component* components;
+} jp2lam_image_view;
+
+/* Decoded Image, library-owned: planes holds num_components planes of
+ * width*height samples (precision 8, unsigned, dx = dy = 1). */
+typedef struct {
+    uint32_t width, height, colorspace, num_components, precision;
+    int32_t* planes;
+} jp2lam_image;
+
+/* DecodeMetadata (src/decode/mod.rs:23-31) flattened. */
+typedef struct {
+    uint32_t width, height, colorspace, num_components, bits_per_component, has_ipr_metadata;
+    uint32_t decomposition_levels, code_block_width, code_block_height, layers, transform, use_mct;
+    uint32_t guard_bits, num_quant_steps;
+    uint64_t tile_part_count, first_tile_payload_len;
+} jp2lam_metadata;
+
+/* ---- crate API ------------------------------------------------------------ */
+
+/* encode(&Image, &EncodeOptions) (src/encode/mod.rs:84-101). */
+jp2lam_status jp2lam_encode(const jp2lam_image_view* image, const jp2lam_encode_options* opts, uint8_t** out,
+                            size_t* out_len);
+/* Image::from_gray_bytes / from_rgb_bytes (src/model.rs:87-140) + encode:
+ * interleaved u8 pixels, channels 1 (gray) or 3 (sRGB). */
+jp2lam_status jp2lam_encode_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                               const jp2lam_encode_options* opts, uint8_t** out, size_t* out_len);
+/* decode_jp2(&[u8]) (src/decode/mod.rs:56-61). */
+jp2lam_status jp2lam_decode_jp2(const uint8_t* bytes, size_t len, jp2lam_image* out);
+/* decode_jp2 into caller-owned interleaved u8 pixels (cap bytes). */
+jp2lam_status jp2lam_decode_jp2_u8(const uint8_t* bytes, size_t len, uint8_t* pixels, size_t cap, uint32_t* width,
+                                   uint32_t* height, uint32_t* channels);
+/* inspect_jp2(&[u8]) (src/decode/mod.rs:38-49). */
+jp2lam_status jp2lam_inspect_jp2(const uint8_t* bytes, size_t len, jp2lam_metadata* out);
+void jp2lam_free(void* p);
+void jp2lam_image_free(jp2lam_image* image);
+const char* jp2lam_last_error(void);
+
+/* ---- batch handles (src/batch.rs:32-130) ------------------------------------
+ * Same contract as BatchEncoder / BatchDecoder: the first item fixes the
+ * profile, later items with another profile fail with INVALID_INPUT.  One
+ * thread per handle. */
+typedef struct jp2lam_batch_encoder jp2lam_batch_encoder;
+typedef struct jp2lam_batch_decoder jp2lam_batch_decoder;
+jp2lam_batch_encoder* jp2lam_batch_encoder_new(const jp2lam_encode_options* opts);
+jp2lam_status jp2lam_batch_encoder_encode_one(jp2lam_batch_encoder* enc, const jp2lam_image_view* image, uint8_t** out,
+                                              size_t* out_len);
+size_t jp2lam_batch_encoder_encoded_count(const jp2lam_batch_encoder* enc);
+int jp2lam_batch_encoder_has_profile(const jp2lam_batch_encoder* enc);
+void jp2lam_batch_encoder_free(jp2lam_batch_encoder* enc);
+jp2lam_batch_decoder* jp2lam_batch_decoder_new(void);
+jp2lam_status jp2lam_batch_decoder_decode_one(jp2lam_batch_decoder* dec, const uint8_t* bytes, size_t len,
+                                              jp2lam_image* out);
+size_t jp2lam_batch_decoder_decoded_count(const jp2lam_batch_decoder* dec);
+void jp2lam_batch_decoder_free(jp2lam_batch_decoder* dec);
+
+/* ---- mixed batches (no counterpart in the reference) ------------------------
+ * Pages of different sizes and qualities in one device batch; each result
+ * equals jp2lam_encode_u8 / jp2lam_decode_jp2_u8 of that item alone.
+ * `on_device` marks pixel pointers that are device (HBM) addresses. */
+typedef struct {
+    const uint8_t* pixels;
+    uint32_t width, height, channels;
+    uint8_t quality, format, on_device, pad;
+    uint8_t* out;      /* set by the library, free with jp2lam_free */
+    size_t out_len;
+    int32_t status;
+} jp2lam_encode_item;
+typedef struct {
+    const uint8_t* bytes;
+    size_t len;
+    uint8_t* pixels;   /* caller buffer, host or device (on_device) */
+    size_t cap;
+    uint32_t width, height, channels;
+    uint8_t on_device, pad[3];
+    int32_t status;
+} jp2lam_decode_item;
+jp2lam_status jp2lam_encode_many(jp2lam_encode_item* items, size_t n);
+jp2lam_status jp2lam_decode_many(jp2lam_decode_item* items, size_t n);
+
+/* ---- stage-level entry points for parity tests ------------------------------- */
+/* Forward colour + DWT: coefficient planes (Mallat layout) as the Tier-1
+ * stage sees them before quantisation: f32 (quality < 100) or i32.  `out`
+ * holds channels * width * height elements. */
+jp2lam_status jp2g_forward_dwt(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                               uint32_t quality, void* out);
+/* Inverse DWT + inverse colour + finalisation of coefficient planes
+ * (f32 dequantised when irreversible, else i32) to interleaved u8. */
+jp2lam_status jp2g_inverse_dwt(const void* planes, uint32_t width, uint32_t height, uint32_t channels,
+                               uint32_t levels, uint32_t irreversible, uint8_t* out);
+/* Tier-1 coding of one i32 code-block (<= 64x64): info = {mbp, zbp, cls,
+ * npasses, nbytes}, passes = 5 ints per pass {kind, bitplane, newly
+ * significant, significant before, length}. */
+jp2lam_status jp2g_t1_encode_block(const int32_t* coeffs, uint32_t w, uint32_t h, uint32_t band, uint32_t mb,
+                                   int32_t* info, int32_t* passes, uint8_t* bytes, size_t cap);
+jp2lam_status jp2g_t1_decode_block(uint32_t w, uint32_t h, uint32_t band, uint32_t mb, uint32_t zbp, uint32_t passes,
+                                   const uint8_t* data, size_t len, int32_t* out);
+/* Per-block record of the last encode while debug capture is on. */
+void jp2g_set_debug(int on);
+int jp2g_last_block_count(void);
+void jp2g_last_block_info(int index, int32_t* info16);
+void jp2g_last_block_passes(int index, int32_t* out);
+void jp2g_last_block_bytes(int index, uint8_t* out);
+double jp2g_last_lambda(int group);
+/* Stage timings of the last call in ms: {dwt, tier1, pcrd+pack, host tier2}. */
+void jp2g_last_timings(double* out4);
+/* Context/MQ tables in the reference's layouts (for table checks). */
+void jp2g_tables(uint16_t* qe, uint8_t* mps, uint8_t* nmps, uint8_t* nlps, uint8_t* zc, uint8_t* sc, uint8_t* spb);
+/* 1 if a HIP device is usable. */
+int jp2g_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

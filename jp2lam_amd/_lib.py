@@ -1,0 +1,111 @@
+"""ctypes binding of ``libjp2lam_amd.so`` (include/jp2lam_amd.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the HIP library is missing or no device is present, the codec
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libjp2lam_amd.so"
+
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+
+
+class EncodeOptionsC(C.Structure):
+    _fields_ = [("quality", C.c_uint8), ("format", C.c_uint8)]
+
+
+class ComponentC(C.Structure):
+    _fields_ = [("data", i32p), ("width", C.c_uint32), ("height", C.c_uint32), ("precision", C.c_uint32),
+                ("is_signed", C.c_uint32), ("dx", C.c_uint32), ("dy", C.c_uint32)]
+
+
+class ImageViewC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("colorspace", C.c_uint32),
+                ("num_components", C.c_uint32), ("components", C.POINTER(ComponentC))]
+
+
+class ImageC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("colorspace", C.c_uint32),
+                ("num_components", C.c_uint32), ("precision", C.c_uint32), ("planes", i32p)]
+
+
+class MetadataC(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "width", "height", "colorspace", "num_components", "bits_per_component", "has_ipr_metadata",
+        "decomposition_levels", "code_block_width", "code_block_height", "layers", "transform", "use_mct",
+        "guard_bits", "num_quant_steps")] + [("tile_part_count", C.c_uint64), ("first_tile_payload_len", C.c_uint64)]
+
+
+class EncodeItemC(C.Structure):
+    _fields_ = [("pixels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32),
+                ("quality", C.c_uint8), ("format", C.c_uint8), ("on_device", C.c_uint8), ("pad", C.c_uint8),
+                ("out", u8p), ("out_len", C.c_size_t), ("status", C.c_int32)]
+
+
+class DecodeItemC(C.Structure):
+    _fields_ = [("bytes", C.c_void_p), ("len", C.c_size_t), ("pixels", C.c_void_p), ("cap", C.c_size_t),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32),
+                ("on_device", C.c_uint8), ("pad", C.c_uint8 * 3), ("status", C.c_int32)]
+
+
+# (name, restype, argtypes) for every symbol the header declares
+SIGNATURES = [
+    ("jp2lam_encode", C.c_int, [C.POINTER(ImageViewC), C.POINTER(EncodeOptionsC), C.POINTER(u8p), C.POINTER(C.c_size_t)]),
+    ("jp2lam_encode_u8", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(EncodeOptionsC),
+                                   C.POINTER(u8p), C.POINTER(C.c_size_t)]),
+    ("jp2lam_decode_jp2", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(ImageC)]),
+    ("jp2lam_decode_jp2_u8", C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("jp2lam_inspect_jp2", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(MetadataC)]),
+    ("jp2lam_free", None, [C.c_void_p]),
+    ("jp2lam_image_free", None, [C.POINTER(ImageC)]),
+    ("jp2lam_last_error", C.c_char_p, []),
+    ("jp2lam_batch_encoder_new", C.c_void_p, [C.POINTER(EncodeOptionsC)]),
+    ("jp2lam_batch_encoder_encode_one", C.c_int, [C.c_void_p, C.POINTER(ImageViewC), C.POINTER(u8p), C.POINTER(C.c_size_t)]),
+    ("jp2lam_batch_encoder_encoded_count", C.c_size_t, [C.c_void_p]),
+    ("jp2lam_batch_encoder_has_profile", C.c_int, [C.c_void_p]),
+    ("jp2lam_batch_encoder_free", None, [C.c_void_p]),
+    ("jp2lam_batch_decoder_new", C.c_void_p, []),
+    ("jp2lam_batch_decoder_decode_one", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(ImageC)]),
+    ("jp2lam_batch_decoder_decoded_count", C.c_size_t, [C.c_void_p]),
+    ("jp2lam_batch_decoder_free", None, [C.c_void_p]),
+    ("jp2lam_encode_many", C.c_int, [C.POINTER(EncodeItemC), C.c_size_t]),
+    ("jp2lam_decode_many", C.c_int, [C.POINTER(DecodeItemC), C.c_size_t]),
+    ("jp2g_forward_dwt", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    ("jp2g_inverse_dwt", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    ("jp2g_t1_encode_block", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("jp2g_t1_decode_block", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.c_void_p, C.c_size_t, C.c_void_p]),
+    ("jp2g_set_debug", None, [C.c_int]),
+    ("jp2g_last_block_count", C.c_int, []),
+    ("jp2g_last_block_info", None, [C.c_int, C.c_void_p]),
+    ("jp2g_last_block_passes", None, [C.c_int, C.c_void_p]),
+    ("jp2g_last_block_bytes", None, [C.c_int, C.c_void_p]),
+    ("jp2g_last_lambda", C.c_double, [C.c_int]),
+    ("jp2g_last_timings", None, [C.c_void_p]),
+    ("jp2g_tables", None, [C.c_void_p] * 7),
+    ("jp2g_device_available", C.c_int, []),
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m jp2lam_amd.build`")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib

@@ -1,0 +1,100 @@
+// Device engine: owns the HIP stream and the device/pinned arenas and runs
+// whole encode/decode batches (one image = a batch of one).
+#pragma once
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+
+namespace j2l {
+
+enum SourceKind : int { SRC_HOST_U8 = 0, SRC_HOST_I32 = 1, SRC_DEV_U8 = 2 };
+
+struct EncItem {
+    uint32_t width = 0, height = 0;
+    int colorspace = GRAY;  // ColorSpace
+    int quality = 85;
+    int format = 0;         // 0 JP2, 1 J2K
+    int src_kind = SRC_HOST_U8;
+    const void* src = nullptr;  // u8 interleaved (gray/RGB) or i32 planar (ncomp planes)
+};
+
+struct EncOut {
+    Status status = OK;
+    std::string message;
+    std::vector<uint8_t> bytes;
+};
+
+enum DecodeTarget : int { DST_HOST_I32 = 0, DST_HOST_U8 = 1, DST_DEV_U8 = 2 };
+
+struct DecItem {
+    const uint8_t* bytes = nullptr;
+    size_t len = 0;
+    int target = DST_HOST_I32;
+    uint8_t* dev_out = nullptr;  // DST_DEV_U8: caller's device buffer (w*h*ncomp bytes)
+    size_t dev_cap = 0;
+};
+
+struct DecOut {
+    Status status = OK;
+    std::string message;
+    uint32_t width = 0, height = 0;
+    int ncomp = 0, colorspace = GRAY;
+    std::vector<int32_t> planes;  // DST_HOST_I32: ncomp planes of width*height
+    std::vector<uint8_t> pixels;  // DST_HOST_U8: interleaved
+};
+
+// Tier-1 block statistics of the last encode (tests and counters).
+struct BlockStat {
+    uint8_t comp, res, band, mbp, zbp, cls, npasses, keep;
+    uint16_t x0, y0, x1, y1;
+    uint32_t nbytes, body_len, max_mag, nnz;
+};
+
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    bool pinned = false;
+    void reserve(size_t bytes);
+    ~Arena();
+};
+
+class Engine {
+public:
+    Engine();
+    ~Engine();
+    std::vector<EncOut> encode(const std::vector<EncItem>& items);
+    std::vector<DecOut> decode(const std::vector<DecItem>& items);
+    // Debug/test access to the last encode: per-block statistics, pass
+    // snapshots and the full Tier-1 segments (before truncation).
+    const std::vector<BlockStat>& last_blocks() const { return last_blocks_; }
+    const std::vector<uint32_t>& last_snaps() const { return last_snaps_; }
+    const std::vector<uint16_t>& last_newsigs() const { return last_newsigs_; }
+    const std::vector<uint8_t>& last_segments() const { return last_segments_; }
+    const std::vector<uint32_t>& last_segment_offsets() const { return last_seg_offs_; }
+    const std::vector<double>& last_lambdas() const { return last_lambdas_; }
+    bool keep_debug = false;
+    // Stage timings of the last call (ms, from HIP events): prepare+DWT,
+    // Tier-1, PCRD+pack, host Tier-2.
+    double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;
+    hipStream_t stream() const { return stream_; }
+    std::mutex mu;
+
+private:
+    hipStream_t stream_ = nullptr;
+    Arena dev_, pin_;
+    uint32_t* tables_ = nullptr;
+    std::vector<BlockStat> last_blocks_;
+    std::vector<uint32_t> last_snaps_;
+    std::vector<uint16_t> last_newsigs_;
+    std::vector<uint8_t> last_segments_;
+    std::vector<uint32_t> last_seg_offs_;
+    std::vector<double> last_lambdas_;
+};
+
+Engine& engine();
+bool device_available(std::string* why = nullptr);
+
+}  // namespace j2l

@@ -1,0 +1,672 @@
+// EBCOT Tier-1 encoder and decoder for CDNA4 (gfx950).
+//
+// One 64-lane wavefront codes one code-block.  Lane x owns column x of the
+// block; its significance, sign, visited and refined state are 64-bit
+// row masks held in VGPRs.  For each 4-row stripe the wave packs, in
+// parallel, a 32-bit neighbourhood word per lane (rows 4s-1..4s+4 of its
+// column), then a wave-uniform loop walks the columns in the reference's
+// scan order (tier1 passes: t1.rs:805-1197, decode/t1.rs:211-466).  The loop
+// carries the west column's new significance from this pass, reads the
+// other columns' words with v_readlane and keeps all coder state in SGPRs,
+// so it runs on the scalar unit while other waves on the SIMD use the VALU.
+// Context and MQ probability tables live one entry per lane in VGPRs and are
+// read with readlane, not memory loads.  Columns with nothing to code are
+// skipped with a ballot mask.
+//
+// The MQ coder/decoder follows mq/mod.rs:30-432 bit for bit, including the
+// byte-position snapshots used as pass lengths (t1.rs:583-600).
+#include "common.hpp"
+
+namespace j2l {
+
+namespace {
+
+constexpr int CTX_ZC = 0, CTX_MAG = 14, CTX_AGG = 17, CTX_UNI = 18;
+
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+// Write a wave-uniform value into one lane of a VGPR (v_cmp + v_cndmask).
+__device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t val, uint32_t lane) {
+    return threadIdx.x == lane ? val : v;
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Rows 4s-1..4s+4 of a column mask as bits 0..5 (missing rows read 0).
+__device__ __forceinline__ uint32_t win6(uint64_t m, int s) {
+    return s ? (uint32_t)(m >> (4 * s - 1)) & 63u : (uint32_t)(m << 1) & 63u;
+}
+
+struct Tables {
+    uint32_t mq;  // lane i < 47: qe | nmps << 16 | nlps << 22 | switch << 28
+    uint32_t zc;  // lane i: ZC contexts of neighbour masks 8i..8i+7 (nibbles)
+    uint32_t sc;  // lane i: sign entries 4i..4i+3 (ctx | spb << 4, bytes)
+};
+
+__device__ __forceinline__ uint32_t zc_ctx(const Tables& t, uint32_t nb) { return (rl(t.zc, nb >> 3) >> ((nb & 7) * 4)) & 15u; }
+__device__ __forceinline__ uint32_t sc_entry(const Tables& t, uint32_t si) { return (rl(t.sc, si >> 2) >> ((si & 3) * 8)) & 255u; }
+
+// 94-state index s = 2 * iso_state + mps; reset values of mq/mod.rs:195-200.
+__device__ __forceinline__ uint32_t init_ctx_lane(int lane) {
+    return lane == CTX_UNI ? 2u * 46 : lane == CTX_AGG ? 2u * 3 : lane == CTX_ZC ? 2u * 4 : 0u;
+}
+
+// ---- MQ encoder (mq/mod.rs:163-432) --------------------------------------
+struct MqEnc {
+    uint32_t a, c, ct, pos, last;
+    uint32_t ctxv;          // VGPR: lane k = state of context k
+    uint32_t win, wbase;    // VGPR byte window for coalesced stores
+    uint8_t* out;
+    uint32_t cap;
+    uint32_t overflow;
+};
+
+__device__ __forceinline__ void enc_commit(MqEnc& m, uint32_t idx, uint32_t byte) {
+    m.win = wl(m.win, byte, idx - m.wbase);
+    if (idx - m.wbase == 63) {
+        uint32_t o = m.wbase + threadIdx.x;
+        if (m.wbase + 64 <= m.cap) m.out[o] = (uint8_t)m.win;
+        else m.overflow = 1;
+        m.wbase += 64;
+    }
+}
+__device__ __forceinline__ void enc_emit(MqEnc& m, uint32_t b) {
+    if (m.pos > 0) enc_commit(m, m.pos - 1, m.last);
+    m.pos++;
+    m.last = b & 0xffu;
+}
+__device__ __forceinline__ void enc_byteout(MqEnc& m) {
+    if (m.last == 0xffu) {
+        enc_emit(m, m.c >> 20);
+        m.c &= 0xfffffu;
+        m.ct = 7;
+    } else if ((m.c & 0x8000000u) == 0) {
+        enc_emit(m, m.c >> 19);
+        m.c &= 0x7ffffu;
+        m.ct = 8;
+    } else {
+        m.last = (m.last + 1) & 0xffu;
+        if (m.last == 0xffu) {
+            m.c &= 0x7ffffffu;
+            enc_emit(m, m.c >> 20);
+            m.c &= 0xfffffu;
+            m.ct = 7;
+        } else {
+            enc_emit(m, m.c >> 19);
+            m.c &= 0x7ffffu;
+            m.ct = 8;
+        }
+    }
+}
+__device__ __forceinline__ void mq_encode(MqEnc& m, const Tables& t, uint32_t ctx, uint32_t bit) {
+    uint32_t s = rl(m.ctxv, ctx);
+    uint32_t e = rl(t.mq, s >> 1);
+    uint32_t qe = e & 0xffffu, mps = s & 1u, ns;
+    m.a -= qe;
+    if (bit == mps) {
+        if (m.a & 0x8000u) {
+            m.c += qe;
+            return;
+        }
+        if (m.a < qe) m.a = qe;
+        else m.c += qe;
+        ns = (((e >> 16) & 63u) << 1) | mps;
+    } else {
+        if (m.a < qe) m.c += qe;
+        else m.a = qe;
+        ns = (((e >> 22) & 63u) << 1) | (mps ^ ((e >> 28) & 1u));
+    }
+    m.ctxv = wl(m.ctxv, ns, ctx);
+    do {
+        m.a <<= 1;
+        m.c <<= 1;
+        if (--m.ct == 0) enc_byteout(m);
+    } while ((m.a & 0x8000u) == 0);
+}
+// flush + take_bytes (mq/mod.rs:244-250, :373-379, :416-423); returns the
+// terminated length.
+__device__ __forceinline__ uint32_t mq_flush(MqEnc& m) {
+    uint32_t temp = m.c + m.a;
+    m.c |= 0xffffu;
+    if (m.c >= temp) m.c -= 0x8000u;
+    m.c <<= m.ct;
+    enc_byteout(m);
+    m.c <<= m.ct;
+    enc_byteout(m);
+    uint32_t total = m.last == 0xffu ? m.pos - 1 : m.pos;
+    if (m.pos > 0) m.win = wl(m.win, m.last, m.pos - 1 - m.wbase);
+    uint32_t nwin = m.pos - m.wbase;  // bytes pending in the window
+    if (threadIdx.x < nwin) {
+        uint32_t o = m.wbase + threadIdx.x;
+        if (o < m.cap) m.out[o] = (uint8_t)m.win;
+        else m.overflow = 1;
+    }
+    if (m.wbase + nwin > m.cap) m.overflow = 1;
+    return total;
+}
+
+// ---- MQ decoder (mq/mod.rs:30-155) -----------------------------------------
+struct MqDec {
+    uint32_t a, c, ct, bp;
+    uint32_t ctxv;
+    uint32_t win, wbase;  // VGPR byte window over the segment
+    const uint8_t* data;
+    uint32_t len;
+};
+__device__ __forceinline__ void dec_load(MqDec& d, uint32_t base) {
+    d.wbase = base;
+    uint32_t i = base + threadIdx.x;
+    d.win = i < d.len ? (uint32_t)d.data[i] : 0xffu;
+}
+__device__ __forceinline__ uint32_t dec_byte(MqDec& d, uint32_t i) {
+    if (i >= d.len) return 0xffu;
+    if (i < d.wbase || i >= d.wbase + 64) dec_load(d, i);
+    return rl(d.win, i - d.wbase);
+}
+__device__ __forceinline__ void dec_bytein(MqDec& d) {
+    uint32_t next = dec_byte(d, d.bp + 1);
+    if (dec_byte(d, d.bp) == 0xffu) {
+        if (next > 0x8fu) {
+            d.c += 0xff00u;
+            d.ct = 8;
+        } else {
+            d.bp++;
+            d.c += next << 9;
+            d.ct = 7;
+        }
+    } else {
+        d.bp++;
+        d.c += next << 8;
+        d.ct = 8;
+    }
+}
+__device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables& t, uint32_t ctx) {
+    uint32_t s = rl(d.ctxv, ctx);
+    uint32_t e = rl(t.mq, s >> 1);
+    uint32_t qe = e & 0xffffu, mps = s & 1u, bit;
+    uint32_t nmps = (((e >> 16) & 63u) << 1) | mps;
+    uint32_t nlps = (((e >> 22) & 63u) << 1) | (mps ^ ((e >> 28) & 1u));
+    d.a -= qe;
+    if ((d.c >> 16) < qe) {
+        uint32_t post = d.a;
+        d.a = qe;
+        if (post < qe) {
+            d.ctxv = wl(d.ctxv, nmps, ctx);
+            bit = mps;
+        } else {
+            d.ctxv = wl(d.ctxv, nlps, ctx);
+            bit = mps ^ 1u;
+        }
+    } else {
+        d.c -= qe << 16;
+        if ((d.a & 0x8000u) == 0) {
+            if (d.a < qe) {
+                d.ctxv = wl(d.ctxv, nlps, ctx);
+                bit = mps ^ 1u;
+            } else {
+                d.ctxv = wl(d.ctxv, nmps, ctx);
+                bit = mps;
+            }
+        } else {
+            bit = mps;
+        }
+    }
+    while (d.a < 0x8000u) {
+        if (d.ct == 0) dec_bytein(d);
+        d.a <<= 1;
+        d.c <<= 1;
+        d.ct--;
+    }
+    return bit;
+}
+
+// ---- shared scan machinery ------------------------------------------------------
+// Per-lane column state.
+struct Col {
+    uint64_t sig, sgn, vis, ref;
+};
+
+// Neighbour mask in the T1_SIGMA layout (tier1/consts.rs:1-27) of row ci of
+// the current column, given effective 6-row windows of W, C (own) and E.
+__device__ __forceinline__ uint32_t nbmask(uint32_t w6, uint32_t c6, uint32_t e6, uint32_t r) {
+    return ((w6 >> (r - 1)) & 1u) | (((c6 >> (r - 1)) & 1u) << 1) | (((e6 >> (r - 1)) & 1u) << 2) |
+           (((w6 >> r) & 1u) << 3) | (((e6 >> r) & 1u) << 5) | (((w6 >> (r + 1)) & 1u) << 6) |
+           (((c6 >> (r + 1)) & 1u) << 7) | (((e6 >> (r + 1)) & 1u) << 8);
+}
+// Sign index (tier1/helpers.rs:53-89 bit layout).
+__device__ __forceinline__ uint32_t signidx(uint32_t w6, uint32_t ws6, uint32_t c6, uint32_t cs6, uint32_t e6,
+                                            uint32_t es6, uint32_t r) {
+    uint32_t lu = 0;
+    if ((w6 >> r) & 1u) lu |= 8u | ((ws6 >> r) & 1u);
+    if ((c6 >> (r - 1)) & 1u) lu |= 2u | (((cs6 >> (r - 1)) & 1u) << 4);
+    if ((e6 >> r) & 1u) lu |= 32u | (((es6 >> r) & 1u) << 2);
+    if ((c6 >> (r + 1)) & 1u) lu |= 128u | (((cs6 >> (r + 1)) & 1u) << 6);
+    return lu;
+}
+
+template <bool DEC>
+struct Coder;
+
+template <>
+struct Coder<false> {
+    MqEnc m;
+    __device__ __forceinline__ uint32_t code(const Tables& t, uint32_t ctx, uint32_t bit) {
+        mq_encode(m, t, ctx, bit);
+        return bit;
+    }
+};
+template <>
+struct Coder<true> {
+    MqDec m;
+    __device__ __forceinline__ uint32_t code(const Tables& t, uint32_t ctx, uint32_t) { return mq_decode(m, t, ctx); }
+};
+
+// Significance propagation pass over the block (t1.rs:805-905,
+// decode/t1.rs:211-266).  `bits` = magnitude bit-plane masks (encoder) and
+// `negs` = sign masks (encoder); the decoder accumulates decoded plane bits
+// into `plane`.  Returns the number of newly significant samples.
+template <bool DEC>
+__device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t negs, uint64_t& plane,
+                            int w, int h) {
+    const int lane = threadIdx.x;
+    uint32_t newsig = 0;
+    const int nstripes = (h + 3) >> 2;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = min(4, h - 4 * s);
+        const uint32_t rmask = (1u << nrows) - 1u;
+        uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
+        uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
+        uint32_t P = sig6 | (sgn6 << 6) | (b4 << 12) | (n4 << 16);
+        // static candidates: not significant with a significant 8-neighbour
+        uint32_t w6 = __shfl_up(sig6, 1, 64), e6 = __shfl_down(sig6, 1, 64);
+        if (lane == 0) w6 = 0;
+        if (lane == 63) e6 = 0;
+        uint32_t o6 = w6 | sig6 | e6;
+        uint32_t nb4 = (o6 | (o6 >> 1) | (o6 >> 2)) & 15u;
+        uint32_t cand = (~(sig6 >> 1)) & nb4 & rmask;
+        uint64_t M = ballot(lane < w && cand != 0);
+        uint32_t upd = 0;
+        uint32_t newW = 0, newWs = 0;
+        int x = M ? __builtin_ctzll(M) : 64;
+        while (x < w) {
+            uint32_t Pc = rl(P, x), Pw = x > 0 ? rl(P, x - 1) : 0u, Pe = x < 63 ? rl(P, x + 1) : 0u;
+            uint32_t sigC = Pc & 63u, sgnC = (Pc >> 6) & 63u, bC = (Pc >> 12) & 15u, nC = (Pc >> 16) & 15u;
+            uint32_t sigW = (Pw & 63u) | (newW << 1), sgnW = ((Pw >> 6) & 63u) | (newWs << 1);
+            uint32_t sigE = Pe & 63u, sgnE = (Pe >> 6) & 63u;
+            uint32_t newC = 0, newCs = 0, visC = 0;
+            for (int ci = 0; ci < nrows; ci++) {
+                const uint32_t r = (uint32_t)ci + 1;
+                if ((sigC >> r) & 1u) continue;
+                uint32_t effC = sigC | (newC << 1);
+                uint32_t nb = nbmask(sigW, effC, sigE, r);
+                if (!nb) continue;
+                uint32_t bit = cd.code(t, zc_ctx(t, nb), (bC >> ci) & 1u);
+                visC |= 1u << ci;
+                if (bit) {
+                    uint32_t effS = sgnC | (newCs << 1);
+                    uint32_t e = sc_entry(t, signidx(sigW, sgnW, effC, effS, sigE, sgnE, r));
+                    uint32_t sg = cd.code(t, e & 15u, ((nC >> ci) & 1u) ^ (e >> 4)) ^ (e >> 4);
+                    newC |= 1u << ci;
+                    newCs |= sg << ci;
+                    newsig++;
+                }
+            }
+            upd = wl(upd, newC | (newCs << 4) | (visC << 8), x);
+            uint64_t rest = x < 63 ? (M >> (x + 1)) << (x + 1) : 0ull;
+            int nx = rest ? __builtin_ctzll(rest) : 64;
+            if (newC && x + 1 < w) nx = x + 1;
+            newW = nx == x + 1 ? newC : 0u;
+            newWs = nx == x + 1 ? newCs : 0u;
+            x = nx;
+        }
+        st.sig |= (uint64_t)(upd & 15u) << (4 * s);
+        st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
+        st.vis |= (uint64_t)((upd >> 8) & 15u) << (4 * s);
+        if (DEC) plane |= (uint64_t)(upd & 15u) << (4 * s);
+    }
+    return newsig;
+}
+
+// Magnitude refinement pass (t1.rs:914-965, decode/t1.rs:268-315): members
+// and contexts are independent of the pass's own decisions, so they are
+// formed lane-parallel; only the coding is serial.
+template <bool DEC>
+__device__ void mr_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t& plane, int w, int h) {
+    const int lane = threadIdx.x;
+    const int nstripes = (h + 3) >> 2;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = min(4, h - 4 * s);
+        const uint32_t rmask = (1u << nrows) - 1u;
+        uint32_t sig6 = win6(st.sig, s);
+        uint32_t sig4 = (sig6 >> 1) & 15u, vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
+        uint32_t ref4 = (uint32_t)(st.ref >> (4 * s)) & 15u, b4 = (uint32_t)(bits >> (4 * s)) & 15u;
+        uint32_t mem = sig4 & ~vis4 & rmask;
+        uint32_t w6 = __shfl_up(sig6, 1, 64), e6 = __shfl_down(sig6, 1, 64);
+        if (lane == 0) w6 = 0;
+        if (lane == 63) e6 = 0;
+        uint32_t Q = mem | (b4 << 12);
+        for (int ci = 0; ci < 4; ci++) {
+            uint32_t r = (uint32_t)ci + 1;
+            uint32_t any = nbmask(w6, sig6, e6, r) != 0;
+            uint32_t off = ((ref4 >> ci) & 1u) ? 2u : any;
+            Q |= off << (4 + 2 * ci);
+        }
+        uint64_t M = ballot(lane < w && mem != 0);
+        uint32_t upd = 0;
+        while (M) {
+            int x = __builtin_ctzll(M);
+            M &= M - 1;
+            uint32_t q = rl(Q, x), ones = 0;
+            for (int ci = 0; ci < nrows; ci++) {
+                if (!((q >> ci) & 1u)) continue;
+                uint32_t bit = cd.code(t, CTX_MAG + ((q >> (4 + 2 * ci)) & 3u), (q >> (12 + ci)) & 1u);
+                ones |= bit << ci;
+            }
+            if (DEC) upd = wl(upd, ones, x);
+        }
+        if (DEC) plane |= (uint64_t)(upd & 15u) << (4 * s);
+    }
+    st.ref |= st.sig & ~st.vis;
+}
+
+// Cleanup pass with run-length mode on full clean stripes (t1.rs:977-1197,
+// decode/t1.rs:317-446).
+template <bool DEC>
+__device__ uint32_t cl_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t negs, uint64_t& plane,
+                            int w, int h) {
+    const int lane = threadIdx.x;
+    uint32_t newsig = 0;
+    const int nstripes = (h + 3) >> 2;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = min(4, h - 4 * s);
+        const uint32_t rmask = (1u << nrows) - 1u;
+        uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
+        uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
+        uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
+        uint32_t w6 = __shfl_up(sig6, 1, 64), e6 = __shfl_down(sig6, 1, 64);
+        if (lane == 0) w6 = 0;
+        if (lane == 63) e6 = 0;
+        // static part of stripe_is_clean (flags.rs:202-215): own rows neither
+        // significant nor visited, no significant neighbour in W/own/E windows
+        uint32_t sclean = (nrows == 4 && sig6 == 0 && vis4 == 0 && w6 == 0 && e6 == 0) ? 1u : 0u;
+        uint32_t P = sig6 | (sgn6 << 6) | (b4 << 12) | (n4 << 16) | (vis4 << 20) | (sclean << 24);
+        uint32_t todo = ~((sig6 >> 1) | vis4) & rmask;
+        uint64_t M = ballot(lane < w && todo != 0);
+        uint32_t upd = 0;
+        uint32_t newW = 0, newWs = 0;
+        int prevx = -2;
+        while (M) {
+            int x = __builtin_ctzll(M);
+            M &= M - 1;
+            if (x != prevx + 1) {
+                newW = 0;
+                newWs = 0;
+            }
+            uint32_t Pc = rl(P, x), Pw = x > 0 ? rl(P, x - 1) : 0u, Pe = x < 63 ? rl(P, x + 1) : 0u;
+            uint32_t sigC = Pc & 63u, sgnC = (Pc >> 6) & 63u, bC = (Pc >> 12) & 15u, nC = (Pc >> 16) & 15u;
+            uint32_t visC = (Pc >> 20) & 15u;
+            uint32_t sigW = (Pw & 63u) | (newW << 1), sgnW = ((Pw >> 6) & 63u) | (newWs << 1);
+            uint32_t sigE = Pe & 63u, sgnE = (Pe >> 6) & 63u;
+            uint32_t newC = 0, newCs = 0;
+            int ci0 = 0;
+            if (((Pc >> 24) & 1u) && newW == 0) {
+                // run mode
+                uint32_t run;
+                if (DEC) {
+                    run = cd.code(t, CTX_AGG, 0);
+                } else {
+                    run = bC ? 1u : 0u;
+                    cd.code(t, CTX_AGG, run);
+                }
+                if (!run) {
+                    ci0 = 4;
+                } else {
+                    uint32_t rlen;
+                    if (DEC) {
+                        uint32_t hi = cd.code(t, CTX_UNI, 0);
+                        uint32_t lo = cd.code(t, CTX_UNI, 0);
+                        rlen = (hi << 1) | lo;
+                    } else {
+                        rlen = (uint32_t)__builtin_ctz(bC);
+                        cd.code(t, CTX_UNI, (rlen >> 1) & 1u);
+                        cd.code(t, CTX_UNI, rlen & 1u);
+                    }
+                    const uint32_t r = rlen + 1;
+                    uint32_t e = sc_entry(t, signidx(sigW, sgnW, sigC, sgnC, sigE, sgnE, r));
+                    uint32_t sg = cd.code(t, e & 15u, ((nC >> rlen) & 1u) ^ (e >> 4)) ^ (e >> 4);
+                    newC = 1u << rlen;
+                    newCs = sg << rlen;
+                    newsig++;
+                    ci0 = (int)rlen + 1;
+                }
+            }
+            for (int ci = ci0; ci < nrows; ci++) {
+                const uint32_t r = (uint32_t)ci + 1;
+                if (((sigC >> r) & 1u) || ((visC >> ci) & 1u)) continue;
+                uint32_t effC = sigC | (newC << 1);
+                uint32_t nb = nbmask(sigW, effC, sigE, r);
+                uint32_t bit = cd.code(t, nb ? zc_ctx(t, nb) : (uint32_t)CTX_ZC, (bC >> ci) & 1u);
+                if (bit) {
+                    uint32_t effS = sgnC | (newCs << 1);
+                    uint32_t e = sc_entry(t, signidx(sigW, sgnW, effC, effS, sigE, sgnE, r));
+                    uint32_t sg = cd.code(t, e & 15u, ((nC >> ci) & 1u) ^ (e >> 4)) ^ (e >> 4);
+                    newC |= 1u << ci;
+                    newCs |= sg << ci;
+                    newsig++;
+                }
+            }
+            if (newC) upd = wl(upd, newC | (newCs << 4), x);
+            newW = newC;
+            newWs = newCs;
+            prevx = x;
+        }
+        st.sig |= (uint64_t)(upd & 15u) << (4 * s);
+        st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
+        if (DEC) plane |= (uint64_t)(upd & 15u) << (4 * s);
+    }
+    return newsig;
+}
+
+__device__ __forceinline__ void load_tables(Tables& t, const uint32_t* __restrict__ tabs, int band) {
+    const int lane = threadIdx.x;
+    t.mq = tabs[lane];
+    t.zc = tabs[64 + band * 64 + lane];
+    t.sc = tabs[320 + lane];
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+
+}  // namespace
+
+// Quantisation of a 9/7 coefficient: trunc(v / step) with IEEE division,
+// non-finite -> 0 (simd/scalar.rs:207-213).
+__device__ __forceinline__ int32_t quantize(float v, float step) {
+    if (!(step > 0.0f) || !isfinite(v)) return 0;
+    float q = truncf(__fdiv_rn(v, step));
+    if (q >= 2147483648.0f) return 0x7fffffff;
+    if (q < -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)q;
+}
+
+// Tier-1 encoder: one wavefront per code-block (blockIdx.x = job index).
+// Outputs the terminated segment into out + job.out_off, the per-pass byte
+// snapshots and newly-significant counts, and the block statistics of
+// t1.rs:215-258.
+__global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+                                                       uint8_t* __restrict__ out, BlockResult* __restrict__ results,
+                                                       uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs) {
+    const int lane = threadIdx.x;
+    const BlockJob J = jobs[blockIdx.x];
+    const int w = J.w, h = J.h;
+    const bool lossy = J.step > 0.0f;
+    uint32_t mag[64];
+    uint64_t neg = 0;
+    uint32_t mx = 0, nz = 0;
+#pragma unroll
+    for (int y = 0; y < 64; y++) {
+        uint32_t m = 0;
+        if (y < h && lane < w) {
+            size_t idx = (size_t)(J.y0 + y) * J.stride + J.x0 + lane;
+            int32_t q = lossy ? quantize(((const float*)J.plane)[idx], J.step) : ((const int32_t*)J.plane)[idx];
+            m = q < 0 ? (uint32_t)(-(int64_t)q) : (uint32_t)q;
+            if (q < 0) neg |= 1ull << y;
+            if (q) nz++;
+        }
+        mag[y] = m;
+        mx = max(mx, m);
+    }
+    mx = wave_max(mx);
+    nz = wave_sum(nz);
+    const int mbp = mx ? 32 - __builtin_clz(mx) : 0;
+    BlockResult R;
+    R.max_mag = mx;
+    R.nnz = (uint16_t)nz;
+    R.mbp = (uint8_t)mbp;
+    R.zbp = (uint8_t)(J.mb > mbp ? J.mb - mbp : 0);
+    float frac = (float)nz / (float)(w * h);  // profile.rs:122-130
+    R.cls = mbp == 0 ? 0 : (frac <= 0.05f ? 0 : (frac <= 0.30f ? 1 : 2));
+    R.overflow = 0;
+    R.pad = 0;
+    R.nbytes = 0;
+    R.npasses = 0;
+    if (mbp == 0) {
+        if (lane == 0) results[blockIdx.x] = R;
+        return;
+    }
+    Tables t;
+    load_tables(t, tabs, J.band);
+    Coder<false> cd;
+    cd.m.a = 0x8000;
+    cd.m.c = 0;
+    cd.m.ct = 12;
+    cd.m.pos = 0;
+    cd.m.last = 0;
+    cd.m.ctxv = init_ctx_lane(lane);
+    cd.m.win = 0;
+    cd.m.wbase = 0;
+    cd.m.out = out + J.out_off;
+    cd.m.cap = J.out_cap;
+    cd.m.overflow = 0;
+    Col st = {0, 0, 0, 0};
+    uint32_t snap0 = 0, snap1 = 0, ns0 = 0, ns1 = 0;
+    int np = 0;
+    const int top = mbp - 1;
+    uint64_t dummy = 0;
+    for (int p = top; p >= 0; p--) {
+        uint64_t bits = 0;
+#pragma unroll
+        for (int y = 0; y < 64; y++) bits |= (uint64_t)((mag[y] >> p) & 1u) << y;
+        if (p != top) {
+            uint32_t n = sp_pass<false>(cd, t, st, bits, neg, dummy, w, h);
+            if (np < 64) { snap0 = wl(snap0, cd.m.pos, np); ns0 = wl(ns0, n, np); }
+            else { snap1 = wl(snap1, cd.m.pos, np - 64); ns1 = wl(ns1, n, np - 64); }
+            np++;
+            mr_pass<false>(cd, t, st, bits, dummy, w, h);
+            if (np < 64) { snap0 = wl(snap0, cd.m.pos, np); ns0 = wl(ns0, 0, np); }
+            else { snap1 = wl(snap1, cd.m.pos, np - 64); ns1 = wl(ns1, 0, np - 64); }
+            np++;
+        }
+        uint32_t n = cl_pass<false>(cd, t, st, bits, neg, dummy, w, h);
+        if (np < 64) { snap0 = wl(snap0, cd.m.pos, np); ns0 = wl(ns0, n, np); }
+        else { snap1 = wl(snap1, cd.m.pos, np - 64); ns1 = wl(ns1, n, np - 64); }
+        np++;
+        st.vis = 0;
+    }
+    uint32_t total = mq_flush(cd.m);
+    R.nbytes = total;
+    R.npasses = (uint8_t)np;
+    R.overflow = (uint8_t)(cd.m.overflow ? 1 : 0);
+    uint32_t* sp = snaps + (size_t)blockIdx.x * kMaxPasses;
+    uint16_t* nsp = newsigs + (size_t)blockIdx.x * kMaxPasses;
+    if (lane < np) {
+        sp[lane] = snap0;
+        nsp[lane] = (uint16_t)ns0;
+    }
+    if (lane + 64 < np && lane + 64 < kMaxPasses) {
+        sp[lane + 64] = snap1;
+        nsp[lane + 64] = (uint16_t)ns1;
+    }
+    if (lane == 0) results[blockIdx.x] = R;
+}
+
+// Tier-1 decoder: one wavefront per code-block.  Decodes `passes` coding
+// passes and writes the reconstructed coefficients into the plane:
+// dequantised f32 (sign * (|q| + 0.5)) * step for 9/7 (scalar.rs:233-240),
+// i32 for 5/3.  Blocks that are not included are written as zeros.
+__global__ __launch_bounds__(64) void t1_decode_kernel(const DecodeJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+                                                       const uint8_t* __restrict__ payload) {
+    const int lane = threadIdx.x;
+    const DecodeJob J = jobs[blockIdx.x];
+    const int w = J.w, h = J.h;
+    const bool lossy = J.step > 0.0f;
+    uint32_t mag[64];
+#pragma unroll
+    for (int y = 0; y < 64; y++) mag[y] = 0;
+    uint64_t sgnmask = 0;
+    if (J.zbp >= 0 && J.passes > 0) {
+        Tables t;
+        load_tables(t, tabs, J.band);
+        Coder<true> cd;
+        cd.m.data = payload + J.data_off;
+        cd.m.len = J.data_len;
+        cd.m.ctxv = init_ctx_lane(lane);
+        dec_load(cd.m, 0);
+        cd.m.bp = 0;
+        cd.m.a = 0x8000;
+        cd.m.c = (J.data_len == 0 ? 0xffu : dec_byte(cd.m, 0)) << 16;
+        cd.m.ct = 0;
+        dec_bytein(cd.m);
+        cd.m.c <<= 7;
+        cd.m.ct = cd.m.ct >= 7 ? cd.m.ct - 7 : 0;
+        Col st = {0, 0, 0, 0};
+        const int mbp = (int)J.mb - (int)J.zbp;
+        int remaining = J.passes;
+        for (int p = mbp - 1; p >= 0 && remaining > 0; p--) {
+            uint64_t plane = 0;
+            if (p != mbp - 1) {
+                sp_pass<true>(cd, t, st, 0, 0, plane, w, h);
+                if (--remaining == 0) goto plane_done;
+                mr_pass<true>(cd, t, st, 0, plane, w, h);
+                if (--remaining == 0) goto plane_done;
+            }
+            cl_pass<true>(cd, t, st, 0, 0, plane, w, h);
+            remaining--;
+        plane_done:
+#pragma unroll
+            for (int y = 0; y < 64; y++) mag[y] |= (uint32_t)((plane >> y) & 1ull) << p;
+            st.vis = 0;
+        }
+        sgnmask = st.sgn;
+    }
+#pragma unroll
+    for (int y = 0; y < 64; y++) {
+        if (y >= h || lane >= w) continue;
+        uint32_t m = mag[y];
+        bool negv = (sgnmask >> y) & 1ull;
+        size_t idx = (size_t)(J.y0 + y) * J.stride + J.x0 + lane;
+        if (lossy) {
+            float v = 0.0f;
+            if (m) v = __fmul_rn(negv ? -__fadd_rn((float)m, 0.5f) : __fadd_rn((float)m, 0.5f), J.step);
+            ((float*)J.plane)[idx] = v;
+        } else {
+            ((int32_t*)J.plane)[idx] = negv ? -(int32_t)m : (int32_t)m;
+        }
+    }
+}
+
+void launch_t1_encode(const BlockJob* jobs, int n, const uint32_t* tabs, uint8_t* out, BlockResult* res, uint32_t* snaps,
+                      uint16_t* newsigs, hipStream_t st) {
+    if (n > 0) t1_encode_kernel<<<n, 64, 0, st>>>(jobs, tabs, out, res, snaps, newsigs);
+}
+void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st) {
+    if (n > 0) t1_decode_kernel<<<n, 64, 0, st>>>(jobs, tabs, payload);
+}
+
+}  // namespace j2l

@@ -463,3 +463,29 @@ int ora_t1_encode_flat(const int32_t *coef, int w, int h, int band, int mb, int3
     free(b.bytes);
     return rc;
 }
+
+/* MQ streams for the coder KATs (mq/mod.rs:213-242, :61-81): encode n
+ * (ctx, bit) symbols, return the terminated length (or -1 if cap is too
+ * small); decode n symbols of the given contexts. */
+int ora_mq_encode_stream(const uint8_t *ctx, const uint8_t *bits, int n, uint8_t *out, int cap) {
+    size_t bcap = (size_t)n * 2 + 64;
+    uint8_t *buf = malloc(bcap);
+    ora_mqc m;
+    ora_mqc_init(&m, buf, bcap);
+    for (int i = 0; i < n; i++) ora_mqc_encode(&m, ctx[i], bits[i]);
+    uint8_t *seg;
+    size_t len = ora_mqc_flush(&m, &seg);
+    int rc = (int)len <= cap ? (int)len : -1;
+    if (rc >= 0) memcpy(out, seg, len);
+    free(buf);
+    return rc;
+}
+void ora_mq_decode_stream(const uint8_t *data, int len, const uint8_t *ctx, int n, uint8_t *bits) {
+    ora_mqd d;
+    ora_mqd_init(&d, data, (size_t)len);
+    for (int i = 0; i < n; i++) bits[i] = (uint8_t)ora_mqd_decode(&d, ctx[i]);
+}
+void ora_tables(uint16_t *qe, uint8_t *mps, uint8_t *nmps, uint8_t *nlps, uint8_t *zc, uint8_t *sc, uint8_t *spb) {
+    ora_mq_state_table(qe, mps, nmps, nlps);
+    ora_t1_luts((uint8_t(*)[512])zc, sc, spb);
+}

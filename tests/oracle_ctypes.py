@@ -1,0 +1,231 @@
+"""ctypes binding of the oracle (oracle/build/liboracle.so).
+
+Test infrastructure only: the oracle is the CPU restatement of the
+reference codec that the parity tests and the CPU baseline compare against.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "build" / "liboracle.so"
+
+vp = C.c_void_p
+
+
+def build_oracle() -> Path:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return ORACLE_LIB
+
+
+class Oracle:
+    def __init__(self, path: Path = ORACLE_LIB):
+        if not path.exists():
+            build_oracle()
+        L = C.CDLL(str(path))
+        sig = {
+            "ora_encode_u8": (vp, [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+            "ora_encode_planes": (vp, [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+            "ora_enc_release": (None, [vp]),
+            "ora_enc_error": (C.c_char_p, [vp]),
+            "ora_enc_file": (C.c_size_t, [vp, C.POINTER(vp)]),
+            "ora_enc_codestream": (C.c_size_t, [vp, C.POINTER(vp)]),
+            "ora_enc_nblocks": (C.c_int, [vp]),
+            "ora_enc_lambda": (C.c_double, [vp, C.c_int]),
+            "ora_enc_plan": (None, [vp, vp]),
+            "ora_enc_block_info": (None, [vp, C.c_int, vp]),
+            "ora_enc_block_passes": (None, [vp, C.c_int, vp]),
+            "ora_enc_block_bytes": (None, [vp, C.c_int, vp]),
+            "ora_coefficients_u8": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, vp]),
+            "ora_decode_u8": (C.c_int, [vp, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_int), vp, C.c_size_t, C.c_char_p]),
+            "ora_t1_encode_flat": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int]),
+            "ora_t1_decode_block": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp,
+                                              C.c_size_t, vp, C.c_char_p]),
+            "ora_mq_encode_stream": (C.c_int, [vp, vp, C.c_int, vp, C.c_int]),
+            "ora_mq_decode_stream": (None, [vp, C.c_int, vp, C.c_int, vp]),
+            "ora_tables": (None, [vp] * 7),
+            "ora_fwd53": (None, [vp, C.c_size_t, C.c_size_t, C.c_int]),
+            "ora_inv53": (None, [vp, C.c_size_t, C.c_size_t, C.c_int]),
+            "ora_fwd97": (None, [vp, C.c_size_t, C.c_size_t, C.c_int]),
+            "ora_inv97": (None, [vp, C.c_size_t, C.c_size_t, C.c_int]),
+            "ora_fwd97_f64": (None, [vp, C.c_size_t, C.c_size_t, C.c_int]),
+            "ora_quantize": (C.c_int32, [C.c_float, C.c_float]),
+            "ora_dequantize": (C.c_float, [C.c_int32, C.c_float]),
+            "ora_inverse_ict": (None, [vp, vp, vp, C.c_size_t]),
+            "ora_inverse_rct": (None, [vp, vp, vp, C.c_size_t]),
+            "ora_forward_ict": (None, [vp, vp, vp, C.c_size_t, C.c_int, vp]),
+            "ora_forward_rct": (None, [vp, vp, vp, C.c_size_t, C.c_int, vp]),
+            "ora_encode_stepsize": (None, [C.c_int32, C.c_uint32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint16)]),
+            "ora_irreversible_expounded_quant": (None, [C.c_uint32, C.c_uint8, C.c_uint8, C.c_int,
+                                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint16)]),
+            "ora_get_norm_97": (C.c_double, [C.c_uint32, C.c_int]),
+            "ora_max_decompositions": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        self.L = L
+
+    # ---- whole codec ----
+    def encode(self, px: np.ndarray, quality: int, fmt: int = 0, keep: bool = False):
+        px = np.ascontiguousarray(px, dtype=np.uint8)
+        h, w = px.shape[:2]
+        ch = 1 if px.ndim == 2 else px.shape[2]
+        st = C.c_int()
+        e = self.L.ora_encode_u8(px.ctypes.data, w, h, ch, quality, fmt, C.byref(st))
+        if st.value:
+            msg = self.L.ora_enc_error(e).decode()
+            self.L.ora_enc_release(e)
+            raise RuntimeError(f"oracle encode failed ({st.value}): {msg}")
+        p = vp()
+        n = self.L.ora_enc_file(e, C.byref(p))
+        data = C.string_at(p, n)
+        if keep:
+            return data, Encoding(self, e)
+        self.L.ora_enc_release(e)
+        return data
+
+    def decode(self, data: bytes) -> np.ndarray:
+        w, h, nc = C.c_uint32(), C.c_uint32(), C.c_int()
+        err = C.create_string_buffer(256)
+        cap = 1 << 20
+        while True:
+            out = np.empty(cap, dtype=np.uint8)
+            rc = self.L.ora_decode_u8(data, len(data), C.byref(w), C.byref(h), C.byref(nc), out.ctypes.data, cap, err)
+            if rc == 5 and b"too small" in err.value:
+                cap = w.value * h.value * max(nc.value, 1) + 16
+                continue
+            if rc:
+                raise OracleError(rc, err.value.decode())
+            n = w.value * h.value * nc.value
+            a = out[:n].copy()
+            return a.reshape(h.value, w.value) if nc.value == 1 else a.reshape(h.value, w.value, nc.value)
+
+    def coefficients(self, px: np.ndarray, quality: int) -> np.ndarray:
+        px = np.ascontiguousarray(px, dtype=np.uint8)
+        h, w = px.shape[:2]
+        ch = 1 if px.ndim == 2 else px.shape[2]
+        out = np.zeros((ch, h, w), dtype=np.int32)
+        assert self.L.ora_coefficients_u8(px.ctypes.data, w, h, ch, quality, out.ctypes.data) == 0
+        return out
+
+    # ---- stages ----
+    def fwd97(self, plane: np.ndarray, levels: int) -> np.ndarray:
+        a = np.ascontiguousarray(plane, dtype=np.float32).copy()
+        self.L.ora_fwd97(a.ctypes.data, a.shape[1], a.shape[0], levels)
+        return a
+
+    def fwd97_f64(self, plane: np.ndarray, levels: int) -> np.ndarray:
+        a = np.ascontiguousarray(plane, dtype=np.float64).copy()
+        self.L.ora_fwd97_f64(a.ctypes.data, a.shape[1], a.shape[0], levels)
+        return a
+
+    def inv97(self, plane: np.ndarray, levels: int) -> np.ndarray:
+        a = np.ascontiguousarray(plane, dtype=np.float32).copy()
+        self.L.ora_inv97(a.ctypes.data, a.shape[1], a.shape[0], levels)
+        return a
+
+    def fwd53(self, plane: np.ndarray, levels: int) -> np.ndarray:
+        a = np.ascontiguousarray(plane, dtype=np.int32).copy()
+        self.L.ora_fwd53(a.ctypes.data, a.shape[1], a.shape[0], levels)
+        return a
+
+    def inv53(self, plane: np.ndarray, levels: int) -> np.ndarray:
+        a = np.ascontiguousarray(plane, dtype=np.int32).copy()
+        self.L.ora_inv53(a.ctypes.data, a.shape[1], a.shape[0], levels)
+        return a
+
+    def t1_encode(self, coeffs: np.ndarray, band: int, mb: int):
+        c = np.ascontiguousarray(coeffs, dtype=np.int32)
+        h, w = c.shape
+        info = np.zeros(5, dtype=np.int32)
+        passes = np.zeros(5 * 96, dtype=np.int32)
+        cap = w * h * 4 + 1024
+        out = np.zeros(cap, dtype=np.uint8)
+        rc = self.L.ora_t1_encode_flat(c.ctypes.data, w, h, band, mb, info.ctypes.data, passes.ctypes.data,
+                                       out.ctypes.data, cap)
+        assert rc == 0
+        n = int(info[3])
+        return info, passes[:5 * n].reshape(n, 5), bytes(out[:info[4]])
+
+    def t1_decode(self, w, h, band, mb, zbp, npasses, data: bytes) -> np.ndarray:
+        out = np.zeros((h, w), dtype=np.int32)
+        err = C.create_string_buffer(256)
+        rc = self.L.ora_t1_decode_block(w, h, band, mb, zbp, npasses, data, len(data), out.ctypes.data, err)
+        if rc:
+            raise OracleError(rc, err.value.decode())
+        return out
+
+    def mq_encode(self, ctx, bits) -> bytes:
+        c = np.ascontiguousarray(ctx, dtype=np.uint8)
+        b = np.ascontiguousarray(bits, dtype=np.uint8)
+        cap = 2 * len(c) + 64
+        out = np.zeros(cap, dtype=np.uint8)
+        n = self.L.ora_mq_encode_stream(c.ctypes.data, b.ctypes.data, len(c), out.ctypes.data, cap)
+        assert n >= 0
+        return bytes(out[:n])
+
+    def mq_decode(self, data: bytes, ctx) -> np.ndarray:
+        c = np.ascontiguousarray(ctx, dtype=np.uint8)
+        out = np.zeros(len(c), dtype=np.uint8)
+        self.L.ora_mq_decode_stream(data, len(data), c.ctypes.data, len(c), out.ctypes.data)
+        return out
+
+    def tables(self):
+        qe = np.zeros(94, np.uint16)
+        mps, nmps, nlps = (np.zeros(94, np.uint8) for _ in range(3))
+        zc = np.zeros(2048, np.uint8)
+        sc, spb = np.zeros(256, np.uint8), np.zeros(256, np.uint8)
+        self.L.ora_tables(*(a.ctypes.data for a in (qe, mps, nmps, nlps, zc, sc, spb)))
+        return qe, mps, nmps, nlps, zc, sc, spb
+
+
+class OracleError(Exception):
+    def __init__(self, code, msg):
+        super().__init__(f"{code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class Encoding:
+    """Per-block view of an oracle encode."""
+
+    def __init__(self, ora: Oracle, handle):
+        self.o = ora
+        self.h = handle
+
+    def __del__(self):
+        if self.h:
+            self.o.L.ora_enc_release(self.h)
+            self.h = None
+
+    @property
+    def nblocks(self) -> int:
+        return self.o.L.ora_enc_nblocks(self.h)
+
+    def info(self, i) -> np.ndarray:
+        a = np.zeros(16, np.int32)
+        self.o.L.ora_enc_block_info(self.h, i, a.ctypes.data)
+        return a
+
+    def passes(self, i) -> np.ndarray:
+        n = int(self.info(i)[11])
+        a = np.zeros(5 * max(n, 1), np.int32)
+        self.o.L.ora_enc_block_passes(self.h, i, a.ctypes.data)
+        return a[:5 * n].reshape(n, 5)
+
+    def bytes(self, i) -> bytes:
+        n = int(self.info(i)[12])
+        a = np.zeros(max(n, 1), np.uint8)
+        self.o.L.ora_enc_block_bytes(self.h, i, a.ctypes.data)
+        return bytes(a[:n])
+
+    def lam(self, c) -> float:
+        return self.o.L.ora_enc_lambda(self.h, c)
