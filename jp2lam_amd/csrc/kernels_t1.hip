@@ -522,8 +522,10 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
         mag[y] = m;
         mx = max(mx, m);
     }
-    mx = wave_max(mx);
-    nz = wave_sum(nz);
+    // The butterfly leaves the same value in every lane; readfirstlane tells
+    // the compiler so, keeping the bit-plane loop and the coder in SGPRs.
+    mx = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_max(mx));
+    nz = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sum(nz));
     const int mbp = mx ? 32 - __builtin_clz(mx) : 0;
     BlockResult R;
     R.max_mag = mx;
