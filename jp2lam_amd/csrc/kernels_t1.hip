@@ -51,100 +51,6 @@ __device__ __forceinline__ uint32_t init_ctx_lane(int lane) {
     return lane == CTX_UNI ? 2u * 46 : lane == CTX_AGG ? 2u * 3 : lane == CTX_ZC ? 2u * 4 : 0u;
 }
 
-// ---- MQ encoder (mq/mod.rs:163-432) --------------------------------------
-struct MqEnc {
-    uint32_t a, c, ct, pos, last;
-    uint32_t ctxv;          // VGPR: lane k = state of context k
-    uint32_t win, wbase;    // VGPR byte window for coalesced stores
-    uint8_t* out;
-    uint32_t cap;
-    uint32_t overflow;
-};
-
-__device__ __forceinline__ void enc_commit(MqEnc& m, uint32_t idx, uint32_t byte) {
-    m.win = wl(m.win, byte, idx - m.wbase);
-    if (idx - m.wbase == 63) {
-        uint32_t o = m.wbase + threadIdx.x;
-        if (m.wbase + 64 <= m.cap) m.out[o] = (uint8_t)m.win;
-        else m.overflow = 1;
-        m.wbase += 64;
-    }
-}
-__device__ __forceinline__ void enc_emit(MqEnc& m, uint32_t b) {
-    if (m.pos > 0) enc_commit(m, m.pos - 1, m.last);
-    m.pos++;
-    m.last = b & 0xffu;
-}
-__device__ __forceinline__ void enc_byteout(MqEnc& m) {
-    if (m.last == 0xffu) {
-        enc_emit(m, m.c >> 20);
-        m.c &= 0xfffffu;
-        m.ct = 7;
-    } else if ((m.c & 0x8000000u) == 0) {
-        enc_emit(m, m.c >> 19);
-        m.c &= 0x7ffffu;
-        m.ct = 8;
-    } else {
-        m.last = (m.last + 1) & 0xffu;
-        if (m.last == 0xffu) {
-            m.c &= 0x7ffffffu;
-            enc_emit(m, m.c >> 20);
-            m.c &= 0xfffffu;
-            m.ct = 7;
-        } else {
-            enc_emit(m, m.c >> 19);
-            m.c &= 0x7ffffu;
-            m.ct = 8;
-        }
-    }
-}
-__device__ __forceinline__ void mq_encode(MqEnc& m, const Tables& t, uint32_t ctx, uint32_t bit) {
-    uint32_t s = rl(m.ctxv, ctx);
-    uint32_t e = rl(t.mq, s >> 1);
-    uint32_t qe = e & 0xffffu, mps = s & 1u, ns;
-    m.a -= qe;
-    if (bit == mps) {
-        if (m.a & 0x8000u) {
-            m.c += qe;
-            return;
-        }
-        if (m.a < qe) m.a = qe;
-        else m.c += qe;
-        ns = (((e >> 16) & 63u) << 1) | mps;
-    } else {
-        if (m.a < qe) m.c += qe;
-        else m.a = qe;
-        ns = (((e >> 22) & 63u) << 1) | (mps ^ ((e >> 28) & 1u));
-    }
-    m.ctxv = wl(m.ctxv, ns, ctx);
-    do {
-        m.a <<= 1;
-        m.c <<= 1;
-        if (--m.ct == 0) enc_byteout(m);
-    } while ((m.a & 0x8000u) == 0);
-}
-// flush + take_bytes (mq/mod.rs:244-250, :373-379, :416-423); returns the
-// terminated length.
-__device__ __forceinline__ uint32_t mq_flush(MqEnc& m) {
-    uint32_t temp = m.c + m.a;
-    m.c |= 0xffffu;
-    if (m.c >= temp) m.c -= 0x8000u;
-    m.c <<= m.ct;
-    enc_byteout(m);
-    m.c <<= m.ct;
-    enc_byteout(m);
-    uint32_t total = m.last == 0xffu ? m.pos - 1 : m.pos;
-    if (m.pos > 0) m.win = wl(m.win, m.last, m.pos - 1 - m.wbase);
-    uint32_t nwin = m.pos - m.wbase;  // bytes pending in the window
-    if (threadIdx.x < nwin) {
-        uint32_t o = m.wbase + threadIdx.x;
-        if (o < m.cap) m.out[o] = (uint8_t)m.win;
-        else m.overflow = 1;
-    }
-    if (m.wbase + nwin > m.cap) m.overflow = 1;
-    return total;
-}
-
 // ---- MQ decoder (mq/mod.rs:30-155) -----------------------------------------
 struct MqDec {
     uint32_t a, c, ct, bp;
@@ -247,14 +153,6 @@ __device__ __forceinline__ uint32_t signidx(uint32_t w6, uint32_t ws6, uint32_t 
 template <bool DEC>
 struct Coder;
 
-template <>
-struct Coder<false> {
-    MqEnc m;
-    __device__ __forceinline__ uint32_t code(const Tables& t, uint32_t ctx, uint32_t bit) {
-        mq_encode(m, t, ctx, bit);
-        return bit;
-    }
-};
 template <>
 struct Coder<true> {
     MqDec m;
@@ -483,6 +381,355 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// ---- encoder: lane-parallel context formation + scalar MQ coder ---------------
+//
+// Each coding pass is formed for the whole block first: for every 4-row
+// stripe, lane x decides which of its four samples are coded, with which
+// context and decision, writes those (context, decision) bytes into LDS at
+// an offset given by a wave prefix sum, and then the scalar MQ loop codes the
+// bytes in order.  The scan order of the reference (stripe, then column,
+// then row) is the byte order in LDS.
+//
+// Symbol byte: context in bits 0..4, decision in bit 7.
+
+constexpr uint32_t kSymCap = 10240;  // 1024 stripe-columns x at most 10 symbols
+
+// v_writelane: write a wave-uniform value into one lane of a VGPR.  The
+// lane select goes through M0 (one SGPR operand per VALU instruction), which
+// is always written by the scalar unit here.
+__device__ __forceinline__ uint32_t wlane(uint32_t v, uint32_t val, uint32_t lane) {
+    asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(val), "{m0}"(lane));
+    return v;
+}
+// Lane i receives lane i-1's value (west neighbour); lane 0 receives 0.
+__device__ __forceinline__ uint32_t from_west(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
+}
+// Lane i receives lane i+1's value (east neighbour); lane 63 receives 0.
+__device__ __forceinline__ uint32_t from_east(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
+}
+// Number of set bits of m in lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// Exclusive prefix sum over lanes of a count < 16, and the wave total.
+__device__ __forceinline__ uint32_t lane_offsets(uint32_t cnt, uint32_t& total) {
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t m = ballot((cnt >> k) & 1u);
+        off += lanes_below(m) << k;
+        tot += (uint32_t)__builtin_popcountll(m) << k;
+    }
+    total = tot;
+    return off;
+}
+__device__ __forceinline__ uint32_t wave_popc4(uint32_t v4) {
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) n += (uint32_t)__builtin_popcountll(ballot((v4 >> k) & 1u));
+    return n;
+}
+// Per-lane table lookups (each lane its own index) through ds_bpermute.
+__device__ __forceinline__ uint32_t zc_lane(const Tables& t, uint32_t nb) {
+    return ((uint32_t)__shfl((int)t.zc, (int)(nb >> 3), 64) >> ((nb & 7u) * 4u)) & 15u;
+}
+__device__ __forceinline__ uint32_t sc_lane(const Tables& t, uint32_t si) {
+    return ((uint32_t)__shfl((int)t.sc, (int)(si >> 2), 64) >> ((si & 3u) * 8u)) & 255u;
+}
+__device__ __forceinline__ uint32_t zc_sym(const Tables& t, uint32_t nb, uint32_t bit) { return zc_lane(t, nb) | (bit << 7); }
+// Sign symbol: context and (sign XOR predicted sign) (t1.rs:884-893).
+__device__ __forceinline__ uint32_t sc_sym(const Tables& t, uint32_t si, uint32_t neg) {
+    const uint32_t e = sc_lane(t, si);
+    return (e & 15u) | ((neg ^ (e >> 4)) << 7);
+}
+
+// Significance propagation pass, context formation (t1.rs:805-905).  A
+// sample is coded when it is insignificant and some neighbour is
+// significant at the time it is reached: neighbours earlier in scan order
+// (the whole west column of the stripe, rows above in its own column) count
+// with this pass's new significance, later ones without.  The new
+// significance of a stripe is the least fixpoint of that rule, found by
+// iterating lane-parallel; each round moves information one column east.
+__device__ uint32_t form_spp(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
+                             uint32_t& newsig) {
+    const bool valid = (int)threadIdx.x < w;
+    const int nstripes = (h + 3) >> 2;
+    uint32_t total = 0, ns = 0;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = min(4, h - 4 * s);
+        const uint32_t rmask = valid ? (1u << nrows) - 1u : 0u;
+        const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
+        const uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
+        const uint32_t w6 = from_west(sig6), e6 = from_east(sig6);
+        const uint32_t ws6 = from_west(sgn6), es6 = from_east(sgn6);
+        const uint32_t x6 = w6 | e6;
+        const uint32_t nbs = (x6 | (x6 >> 1) | (x6 >> 2) | sig6 | (sig6 >> 2)) & 15u;
+        const uint32_t uns = ~(sig6 >> 1) & rmask;
+        uint32_t nsig = 0, coded;
+        for (;;) {
+            const uint32_t wn = from_west(nsig);
+            uint32_t c = uns & (nbs | ((wn | (wn << 1) | (wn >> 1)) & 15u));
+            uint32_t n = c & b4;
+            c |= (n << 1) & uns;
+            n = c & b4;
+            c |= (n << 1) & uns;
+            n = c & b4;
+            c |= (n << 1) & uns;
+            n = c & b4;
+            coded = c;
+            const bool grew = n != nsig;
+            nsig = n;
+            if (!ballot(grew)) break;
+        }
+        const uint32_t nsgn = nsig & n4;
+        const uint32_t wE = w6 | (from_west(nsig) << 1), wsE = ws6 | (from_west(nsgn) << 1);
+        const uint32_t cnt = (uint32_t)__builtin_popcount(coded) + (uint32_t)__builtin_popcount(nsig);
+        uint32_t tot;
+        uint32_t pos = total + lane_offsets(cnt, tot);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t above = (1u << r) - 1u;
+            const uint32_t cE = sig6 | ((nsig & above) << 1), csE = sgn6 | ((nsgn & above) << 1);
+            const uint32_t zs = zc_sym(t, nbmask(wE, cE, e6, r + 1), (b4 >> r) & 1u);
+            const uint32_t ss = sc_sym(t, signidx(wE, wsE, cE, csE, e6, es6, r + 1), (n4 >> r) & 1u);
+            if ((coded >> r) & 1u) {
+                symb[pos++] = (uint8_t)zs;
+                if ((nsig >> r) & 1u) symb[pos++] = (uint8_t)ss;
+            }
+        }
+        total += tot;
+        ns += wave_popc4(nsig);
+        st.sig |= (uint64_t)nsig << (4 * s);
+        st.sgn |= (uint64_t)nsgn << (4 * s);
+        st.vis |= (uint64_t)coded << (4 * s);
+    }
+    newsig = ns;
+    return total;
+}
+
+// Magnitude refinement pass, context formation (t1.rs:914-965): members are
+// significant samples not coded in this plane's SP pass; the context is 16
+// once refined, else 15/14 by any significant 8-neighbour.
+__device__ uint32_t form_mrp(Col& st, uint64_t bits, int w, int h, uint8_t* symb) {
+    const bool valid = (int)threadIdx.x < w;
+    const int nstripes = (h + 3) >> 2;
+    uint32_t total = 0;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = min(4, h - 4 * s);
+        const uint32_t rmask = valid ? (1u << nrows) - 1u : 0u;
+        const uint32_t sig6 = win6(st.sig, s);
+        const uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u, ref4 = (uint32_t)(st.ref >> (4 * s)) & 15u;
+        const uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u;
+        const uint32_t mem = (sig6 >> 1) & ~vis4 & rmask;
+        const uint32_t w6 = from_west(sig6), e6 = from_east(sig6);
+        uint32_t tot;
+        uint32_t pos = total + lane_offsets((uint32_t)__builtin_popcount(mem), tot);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if ((mem >> r) & 1u) {
+                const uint32_t ctx = ((ref4 >> r) & 1u) ? CTX_MAG + 2u : (nbmask(w6, sig6, e6, r + 1) ? CTX_MAG + 1u : CTX_MAG);
+                symb[pos++] = (uint8_t)(ctx | (((b4 >> r) & 1u) << 7));
+            }
+        }
+        total += tot;
+    }
+    st.ref |= st.sig & ~st.vis;
+    return total;
+}
+
+// Cleanup pass, context formation (t1.rs:977-1197): every insignificant,
+// unvisited sample is coded, so the pass's new significance is simply
+// bit & todo.  Full stripes that are clean (flags.rs:202-215) and have no
+// new significance in the west column use run mode: AGG, then if any bit is
+// set the run length as two UNI decisions and the first sample's sign.
+__device__ uint32_t form_cup(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
+                             uint32_t& newsig) {
+    const bool valid = (int)threadIdx.x < w;
+    const int nstripes = (h + 3) >> 2;
+    uint32_t total = 0, ns = 0;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = min(4, h - 4 * s);
+        const uint32_t rmask = valid ? (1u << nrows) - 1u : 0u;
+        const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
+        const uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
+        const uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
+        const uint32_t w6 = from_west(sig6), e6 = from_east(sig6);
+        const uint32_t ws6 = from_west(sgn6), es6 = from_east(sgn6);
+        const uint32_t todo = ~((sig6 >> 1) | vis4) & rmask;
+        const uint32_t nsig = todo & b4, nsgn = nsig & n4;
+        const uint32_t wn = from_west(nsig);
+        const uint32_t wE = w6 | (wn << 1), wsE = ws6 | (from_west(nsgn) << 1);
+        const bool run = valid && nrows == 4 && sig6 == 0 && vis4 == 0 && w6 == 0 && e6 == 0 && wn == 0;
+        const uint32_t rl0 = b4 ? (uint32_t)__builtin_ctz(b4) : 4u;
+        const uint32_t reg = run ? (b4 ? todo & ~((2u << rl0) - 1u) : 0u) : todo;
+        const uint32_t hdr = run ? (b4 ? 4u : 1u) : 0u;
+        const uint32_t cnt = hdr + (uint32_t)__builtin_popcount(reg) + (uint32_t)__builtin_popcount(nsig & reg);
+        uint32_t tot;
+        uint32_t pos = total + lane_offsets(cnt, tot);
+        // run-mode sign of the first set row: nothing above it is significant
+        const uint32_t rs = sc_sym(t, signidx(wE, wsE, sig6, sgn6, e6, es6, rl0 + 1), (n4 >> (rl0 & 3u)) & 1u);
+        if (run) {
+            symb[pos] = (uint8_t)(CTX_AGG | (b4 ? 128u : 0u));
+            if (b4) {
+                symb[pos + 1] = (uint8_t)(CTX_UNI | (((rl0 >> 1) & 1u) << 7));
+                symb[pos + 2] = (uint8_t)(CTX_UNI | ((rl0 & 1u) << 7));
+                symb[pos + 3] = (uint8_t)rs;
+            }
+            pos += hdr;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t above = (1u << r) - 1u;
+            const uint32_t cE = sig6 | ((nsig & above) << 1), csE = sgn6 | ((nsgn & above) << 1);
+            const uint32_t zs = zc_sym(t, nbmask(wE, cE, e6, r + 1), (b4 >> r) & 1u);
+            const uint32_t ss = sc_sym(t, signidx(wE, wsE, cE, csE, e6, es6, r + 1), (n4 >> r) & 1u);
+            if ((reg >> r) & 1u) {
+                symb[pos++] = (uint8_t)zs;
+                if ((nsig >> r) & 1u) symb[pos++] = (uint8_t)ss;
+            }
+        }
+        total += tot;
+        ns += wave_popc4(nsig);
+        st.sig |= (uint64_t)nsig << (4 * s);
+        st.sgn |= (uint64_t)nsgn << (4 * s);
+    }
+    newsig = ns;
+    return total;
+}
+
+// ---- MQ encoder (mq/mod.rs:163-432) --------------------------------------------
+// Coder registers in SGPRs.  cx lane k holds context k's current state as
+// its table entry (qe | nmps << 16 | nlps << 22 | switch << 28) with the
+// MPS in bit 31, so the common MPS path needs one readlane.  Output bytes
+// collect in a 64-byte VGPR window that is stored with one coalesced write.
+struct MqE {
+    uint32_t a, c, ct, pos, last;
+    uint32_t cx, tab;       // VGPRs
+    uint32_t win, wbase;    // VGPR byte window
+    uint8_t* out;
+    uint32_t cap, overflow;
+};
+
+__device__ __forceinline__ void mqe_emit(MqE& m, uint32_t b) {
+    if (m.pos > 0) {
+        const uint32_t idx = m.pos - 1u - m.wbase;
+        m.win = wlane(m.win, m.last, idx);
+        if (idx == 63u) {
+            if (m.wbase + 64u <= m.cap) m.out[m.wbase + threadIdx.x] = (uint8_t)m.win;
+            else m.overflow = 1u;
+            m.wbase += 64u;
+        }
+    }
+    m.pos++;
+    m.last = b & 0xffu;
+}
+__device__ __forceinline__ void mqe_byteout(MqE& m) {
+    if (m.last == 0xffu) {
+        mqe_emit(m, m.c >> 20);
+        m.c &= 0xfffffu;
+        m.ct = 7;
+    } else if ((m.c & 0x8000000u) == 0) {
+        mqe_emit(m, m.c >> 19);
+        m.c &= 0x7ffffu;
+        m.ct = 8;
+    } else {
+        m.last = (m.last + 1u) & 0xffu;
+        if (m.last == 0xffu) {
+            m.c &= 0x7ffffffu;
+            mqe_emit(m, m.c >> 20);
+            m.c &= 0xfffffu;
+            m.ct = 7;
+        } else {
+            mqe_emit(m, m.c >> 19);
+            m.c &= 0x7ffffu;
+            m.ct = 8;
+        }
+    }
+}
+// Renormalising path of one symbol (LPS, or MPS that leaves a < 0x8000):
+// conditional exchange, state transition, then shift until a >= 0x8000,
+// emitting a byte whenever ct runs out.
+__device__ __forceinline__ void mqe_slow(MqE& m, uint32_t e, uint32_t qe, uint32_t a, uint32_t ctx, bool is_mps) {
+    const bool to_qe = (a < qe) == is_mps;
+    m.a = to_qe ? qe : a;
+    m.c += to_qe ? 0u : qe;
+    const uint32_t next = is_mps ? (e >> 16) & 63u : (e >> 22) & 63u;
+    const uint32_t mps = is_mps ? e : e ^ (e << 3);
+    m.cx = wlane(m.cx, rl(m.tab, next) | (mps & 0x80000000u), ctx);
+    uint32_t sh = (uint32_t)__builtin_clz(m.a) - 16u;
+    if (sh < m.ct) {
+        m.a <<= sh;
+        m.c <<= sh;
+        m.ct -= sh;
+    } else {
+        do {
+            m.a <<= m.ct;
+            m.c <<= m.ct;
+            sh -= m.ct;
+            mqe_byteout(m);
+        } while (sh >= m.ct);
+        m.a <<= sh;
+        m.c <<= sh;
+        m.ct -= sh;
+    }
+}
+// One symbol; `d31` carries the decision in bit 31 (other bits ignored).
+// The common case (MPS with a still >= 0x8000) is tested with integer ops:
+// bit 31 of (a << 16) & ~(entry ^ d31) is set exactly then.
+__device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
+    const uint32_t e = rl(m.cx, ctx);
+    const uint32_t qe = e & 0xffffu;
+    const uint32_t a = m.a - qe;
+    const uint32_t x = e ^ d31;
+    if (__builtin_expect((int32_t)((a << 16) & ~x) < 0, 1)) {
+        m.a = a;
+        m.c += qe;
+    } else {
+        mqe_slow(m, e, qe, a, ctx, (int32_t)x >= 0);
+    }
+}
+// Code the n symbol bytes of one pass from LDS.
+__device__ __forceinline__ void mqe_pass(MqE& m, const uint32_t* symw, uint32_t n) {
+    for (uint32_t base = 0; base < n; base += 256u) {
+        const uint32_t sv = symw[(base >> 2) + threadIdx.x];
+        const uint32_t cnt = min(256u, n - base);
+        uint32_t q = 0;
+        for (; q + 4u <= cnt; q += 4u) {
+            const uint32_t w = rl(sv, q >> 2);
+            mqe_symbol(m, w & 31u, w << 24);
+            mqe_symbol(m, (w >> 8) & 31u, w << 16);
+            mqe_symbol(m, (w >> 16) & 31u, w << 8);
+            mqe_symbol(m, (w >> 24) & 31u, w);
+        }
+        if (q < cnt) {
+            uint32_t w = rl(sv, q >> 2);
+            for (; q < cnt; q++, w >>= 8) mqe_symbol(m, w & 31u, w << 24);
+        }
+    }
+}
+// flush + take_bytes (mq/mod.rs:244-250, :373-379, :416-423); returns the
+// terminated length.
+__device__ __forceinline__ uint32_t mqe_flush(MqE& m) {
+    uint32_t temp = m.c + m.a;
+    m.c |= 0xffffu;
+    if (m.c >= temp) m.c -= 0x8000u;
+    m.c <<= m.ct;
+    mqe_byteout(m);
+    m.c <<= m.ct;
+    mqe_byteout(m);
+    const uint32_t total = m.last == 0xffu ? m.pos - 1u : m.pos;
+    if (m.pos > 0) m.win = wlane(m.win, m.last, m.pos - 1u - m.wbase);
+    const uint32_t nwin = m.pos - m.wbase;
+    if (threadIdx.x < nwin) {
+        const uint32_t o = m.wbase + threadIdx.x;
+        if (o < m.cap) m.out[o] = (uint8_t)m.win;
+    }
+    if (m.wbase + nwin > m.cap) m.overflow = 1u;
+    return total;
+}
+
 }  // namespace
 
 // Quantisation of a 9/7 coefficient: trunc(v / step) with IEEE division,
@@ -502,6 +749,8 @@ __device__ __forceinline__ int32_t quantize(float v, float step) {
 __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                        uint8_t* __restrict__ out, BlockResult* __restrict__ results,
                                                        uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs) {
+    __shared__ uint32_t symw[kSymCap / 4];
+    uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
     const int lane = threadIdx.x;
     const BlockJob J = jobs[blockIdx.x];
     const int w = J.w, h = J.h;
@@ -544,54 +793,69 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     }
     Tables t;
     load_tables(t, tabs, J.band);
-    Coder<false> cd;
-    cd.m.a = 0x8000;
-    cd.m.c = 0;
-    cd.m.ct = 12;
-    cd.m.pos = 0;
-    cd.m.last = 0;
-    cd.m.ctxv = init_ctx_lane(lane);
-    cd.m.win = 0;
-    cd.m.wbase = 0;
-    cd.m.out = out + J.out_off;
-    cd.m.cap = J.out_cap;
-    cd.m.overflow = 0;
+    MqE m;
+    m.a = 0x8000u;
+    m.c = 0;
+    m.ct = 12;
+    m.pos = 0;
+    m.last = 0;
+    m.tab = t.mq;
+    // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
+    m.cx = (uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64);
+    m.win = 0;
+    m.wbase = 0;
+    m.out = out + J.out_off;
+    m.cap = J.out_cap;
+    m.overflow = 0;
     Col st = {0, 0, 0, 0};
     uint32_t snap0 = 0, snap1 = 0, ns0 = 0, ns1 = 0;
-    int np = 0;
+    uint32_t np = 0;
+    auto record = [&](uint32_t n) {
+        if (np < 64u) {
+            snap0 = wlane(snap0, m.pos, np);
+            ns0 = wlane(ns0, n, np);
+        } else {
+            snap1 = wlane(snap1, m.pos, np - 64u);
+            ns1 = wlane(ns1, n, np - 64u);
+        }
+        np++;
+    };
     const int top = mbp - 1;
-    uint64_t dummy = 0;
     for (int p = top; p >= 0; p--) {
         uint64_t bits = 0;
 #pragma unroll
         for (int y = 0; y < 64; y++) bits |= (uint64_t)((mag[y] >> p) & 1u) << y;
+        uint32_t ns, n;
         if (p != top) {
-            uint32_t n = sp_pass<false>(cd, t, st, bits, neg, dummy, w, h);
-            if (np < 64) { snap0 = wl(snap0, cd.m.pos, np); ns0 = wl(ns0, n, np); }
-            else { snap1 = wl(snap1, cd.m.pos, np - 64); ns1 = wl(ns1, n, np - 64); }
-            np++;
-            mr_pass<false>(cd, t, st, bits, dummy, w, h);
-            if (np < 64) { snap0 = wl(snap0, cd.m.pos, np); ns0 = wl(ns0, 0, np); }
-            else { snap1 = wl(snap1, cd.m.pos, np - 64); ns1 = wl(ns1, 0, np - 64); }
-            np++;
+            n = form_spp(t, st, bits, neg, w, h, symb, ns);
+            __syncthreads();
+            mqe_pass(m, symw, n);
+            record(ns);
+            __syncthreads();
+            n = form_mrp(st, bits, w, h, symb);
+            __syncthreads();
+            mqe_pass(m, symw, n);
+            record(0);
+            __syncthreads();
         }
-        uint32_t n = cl_pass<false>(cd, t, st, bits, neg, dummy, w, h);
-        if (np < 64) { snap0 = wl(snap0, cd.m.pos, np); ns0 = wl(ns0, n, np); }
-        else { snap1 = wl(snap1, cd.m.pos, np - 64); ns1 = wl(ns1, n, np - 64); }
-        np++;
+        n = form_cup(t, st, bits, neg, w, h, symb, ns);
+        __syncthreads();
+        mqe_pass(m, symw, n);
+        record(ns);
+        __syncthreads();
         st.vis = 0;
     }
-    uint32_t total = mq_flush(cd.m);
+    const uint32_t total = mqe_flush(m);
     R.nbytes = total;
     R.npasses = (uint8_t)np;
-    R.overflow = (uint8_t)(cd.m.overflow ? 1 : 0);
+    R.overflow = (uint8_t)(m.overflow ? 1 : 0);
     uint32_t* sp = snaps + (size_t)blockIdx.x * kMaxPasses;
     uint16_t* nsp = newsigs + (size_t)blockIdx.x * kMaxPasses;
-    if (lane < np) {
+    if ((uint32_t)lane < np) {
         sp[lane] = snap0;
         nsp[lane] = (uint16_t)ns0;
     }
-    if (lane + 64 < np && lane + 64 < kMaxPasses) {
+    if ((uint32_t)lane + 64u < np && lane + 64 < kMaxPasses) {
         sp[lane + 64] = snap1;
         nsp[lane + 64] = (uint16_t)ns1;
     }
