@@ -31,6 +31,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the post-run oracle check")
+    ap.add_argument("--batch-pages", type=int, default=8,
+                    help="pages in the batched-encode measurement (0 = skip)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB (a known-size kernel "
+                         "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -115,6 +120,58 @@ def main():
     stage_ms = {k: round(v[0], 4) for k, v in stages.items()}
     stage_ms["encode_host_tier2"] = round(enc_t[3], 4)
 
+    # ---- plain encode only, then batched encode of distinct pages ----------
+    def enc_plain():
+        r = j.encode_many([((d_in.data_ptr(), H, W, 1), Q)], device=True)[0]
+        if isinstance(r, Exception):
+            raise r
+        return r
+
+    batched = None
+    if args.batch_pages > 0:
+        enc_plain()
+        torch.cuda.synchronize()
+        reps = max(3, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            enc_plain()
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1000.0 / reps
+        nb = args.batch_pages
+        pages = [torch.from_numpy(synth.document_page(W, H, seed=2 + 1000 * k)).to(dev) for k in range(nb)]
+        items = [((pg.data_ptr(), H, W, 1), Q) for pg in pages]
+        torch.cuda.synchronize()
+        res = j.encode_many(items, device=True)
+        bad = [r for r in res if isinstance(r, Exception)]
+        if bad:
+            raise bad[0]
+        breps = max(2, reps // 2)
+        t0 = time.perf_counter()
+        for _ in range(breps):
+            j.encode_many(items, device=True)
+        torch.cuda.synchronize()
+        batch_ms = (time.perf_counter() - t0) * 1000.0 / breps
+        tb = np.zeros(4, np.float64)
+        lib.jp2g_last_timings(tb.ctypes.data)
+        batched = {"pages": nb, "t1_coder": "lane per block" if lib.jp2g_last_t1_lanes() else "wave per block",
+                   "encode_ms_per_batch": round(batch_ms, 3),
+                   "encode_mpx_s": round(nb * W * H / 1e6 / (batch_ms / 1e3), 2),
+                   "plain_encode_ms_per_page": round(plain_ms, 3),
+                   "plain_encode_mpx_s": round(W * H / 1e6 / (plain_ms / 1e3), 2),
+                   "speedup_vs_plain": round(plain_ms * nb / batch_ms, 3),
+                   "stage_ms": {"dwt": round(tb[0], 3), "t1_encode": round(tb[1], 3), "pcrd_pack": round(tb[2], 3),
+                                "host_tier2": round(tb[3], 3)}}
+        del pages
+
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib << 20
+        a = torch.ones(n, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     # ---- parity check of the measured output (outside the timed region) ----
     check = None
     if not args.no_check:
@@ -167,6 +224,7 @@ def main():
         "cpu_baseline": cpu,
         "stage_ms": stage_ms,
         "parity_vs_oracle": check,
+        "batched_encode": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -169,6 +169,14 @@ void jp2g_last_block_info(int index, int32_t* info16);
 void jp2g_last_block_passes(int index, int32_t* out);
 void jp2g_last_block_bytes(int index, uint8_t* out);
 double jp2g_last_lambda(int group);
+/* Tier-1 encoder selection (no reference counterpart; the reference codes
+ * every block the same way, t1.rs:287-290): mode 0 = by batch size, the
+ * lane-per-block coder from `lane_min_blocks` code-blocks up; 1 = one
+ * wavefront per code-block; 2 = one lane per code-block.  Output is the
+ * same in every mode. */
+void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks);
+/* 1 when the last encode ran the lane-per-block coder. */
+int jp2g_last_t1_lanes(void);
 /* Stage timings of the last call in ms: {dwt, tier1, pcrd+pack, host tier2}. */
 void jp2g_last_timings(double* out4);
 /* Context/MQ tables in the reference's layouts (for table checks). */

@@ -662,6 +662,13 @@ jp2lam_status jp2g_t1_decode_block(uint32_t w, uint32_t h, uint32_t band, uint32
 }
 
 void jp2g_set_debug(int on) { engine().keep_debug = on != 0; }
+void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks) {
+    Engine& e = engine();
+    std::lock_guard<std::mutex> g(e.mu);
+    e.t1_mode = mode < 0 || mode > 2 ? 0 : mode;
+    if (lane_min_blocks) e.lane_min_blocks = lane_min_blocks;
+}
+int jp2g_last_t1_lanes(void) { return engine().last_t1_lanes ? 1 : 0; }
 int jp2g_last_block_count(void) { return (int)engine().last_blocks().size(); }
 void jp2g_last_block_info(int i, int32_t* info) {
     const BlockStat& b = engine().last_blocks()[i];

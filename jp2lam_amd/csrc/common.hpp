@@ -129,6 +129,12 @@ void launch_prepare(const PixSeg* segs, int nseg, uint32_t nblocks, hipStream_t 
 void launch_finalize(const PixSeg* segs, int nseg, uint32_t nblocks, hipStream_t st);
 void launch_t1_encode(const BlockJob* jobs, int n, const uint32_t* tabs, uint8_t* out, BlockResult* res, uint32_t* snaps,
                       uint16_t* newsigs, hipStream_t st);
+void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResult* res, uint32_t* pend, uint16_t* newsigs,
+                    uint32_t* soff16, uint8_t* pool, unsigned long long* pool_used, unsigned long long pool_cap,
+                    hipStream_t st);
+void launch_t1_mq(const BlockJob* jobs, int n, const uint32_t* tabs, const uint8_t* pool, const uint32_t* soff16,
+                  const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, hipStream_t st);
+uint64_t t1_stream_bound(uint32_t mbp);
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st);
 void launch_pcrd(int n, bool lossless, int ngroups, const BlockResult* res, const uint32_t* snaps,
                  const uint16_t* newsigs, const PcrdJob* jobs, HullPt* hull, uint8_t* hull_n, unsigned long long* gmax,

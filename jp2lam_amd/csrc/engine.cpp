@@ -262,6 +262,19 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     const size_t o_blkimg = L.add(4 * nb);
     const size_t o_packbase = L.add(8 * imgs.size());
     const size_t o_packed = L.add(pack_total);
+    // Lane-per-block Tier-1 (large batches): symbol streams, bounded per
+    // block by the band's Mb + 2 bit-planes; an overflow falls back to the
+    // wave-per-block kernel below.
+    const bool lanes = t1_mode == 2 || (t1_mode == 0 && (uint32_t)nb >= lane_min_blocks);
+    size_t o_pend = 0, o_soff = 0, o_pused = 0, o_pool = 0;
+    unsigned long long pool_cap = 0;
+    if (lanes) {
+        for (auto& j : jobs) pool_cap += t1_stream_bound((uint32_t)j.mb + 2u);
+        o_pend = L.add(sizeof(uint32_t) * kMaxPasses * nb);
+        o_soff = L.add(sizeof(uint32_t) * nb);
+        o_pused = L.add(8);
+        o_pool = L.add(pool_cap + 64);
+    }
 
     // Segment tables, split by sample type (f32 for 9/7, i32 for 5/3) so
     // every launch is homogeneous.  Addresses are arena offsets until the
@@ -378,9 +391,22 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         if (im.plan.levels == 0)
             for (int c = 0; c < im.plan.ncomp; c++)
                 J2L_HIP(hipMemcpyAsync(D + im.p_off[c], D + im.x_off[c], im.n * 4, hipMemcpyDeviceToDevice, st));
+    const unsigned long long zero64 = 0;
+    bool use_lanes = lanes;
+t1_again:
     J2L_HIP(hipEventRecord(ev.e[1], st));
-    launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
-                     (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
+    if (use_lanes) {
+        J2L_HIP(hipMemcpyAsync(D + o_pused, &zero64, 8, hipMemcpyHostToDevice, st));
+        launch_t1_form((const BlockJob*)(D + o_jobs), nb, tables_, (BlockResult*)(D + o_res), (uint32_t*)(D + o_pend),
+                       (uint16_t*)(D + o_ns), (uint32_t*)(D + o_soff), (uint8_t*)(D + o_pool),
+                       (unsigned long long*)(D + o_pused), pool_cap, st);
+        launch_t1_mq((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
+                     (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
+                     (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), st);
+    } else {
+        launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
+                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
+    }
     J2L_HIP(hipEventRecord(ev.e[2], st));
     // PCRD: lossless images keep every pass; lossy ones go through the hull.
     bool any_lossy = false, any_lossless = false;
@@ -428,6 +454,15 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     t_pcrd_ms = ev.ms(2, 3);
     const Selection* sel = (const Selection*)(P + h_sel);
     const BlockResult* res = (const BlockResult*)(P + h_res);
+    if (use_lanes) {
+        bool pool_full = false;
+        for (int b = 0; b < nb && !pool_full; b++) pool_full = res[b].overflow == 2;
+        if (pool_full) {
+            use_lanes = false;
+            goto t1_again;
+        }
+    }
+    last_t1_lanes = use_lanes;
     const uint32_t* totals = (const uint32_t*)(P + h_tot);
     std::vector<Selection> selv(sel, sel + nb);
     std::vector<BlockResult> resv(res, res + nb);

@@ -76,6 +76,11 @@ public:
     const std::vector<uint32_t>& last_segment_offsets() const { return last_seg_offs_; }
     const std::vector<double>& last_lambdas() const { return last_lambdas_; }
     bool keep_debug = false;
+    // Tier-1 encoder choice: 0 = by batch size (lane-per-block coder from
+    // lane_min_blocks code-blocks up), 1 = wave per block, 2 = lane per block.
+    int t1_mode = 0;
+    uint32_t lane_min_blocks = 4000;
+    bool last_t1_lanes = false;
     // Stage timings of the last call (ms, from HIP events): prepare+DWT,
     // Tier-1, PCRD+pack, host Tier-2.
     double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;

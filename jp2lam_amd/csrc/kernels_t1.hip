@@ -398,6 +398,10 @@ constexpr uint32_t kSymCap = 10240;  // 1024 stripe-columns x at most 10 symbols
 // lane select goes through M0 (one SGPR operand per VALU instruction), which
 // is always written by the scalar unit here.
 __device__ __forceinline__ uint32_t wlane(uint32_t v, uint32_t val, uint32_t lane) {
+    // val and lane are wave-uniform by contract; readfirstlane states it
+    // for the compiler (free when they already sit in SGPRs).
+    val = (uint32_t)__builtin_amdgcn_readfirstlane((int)val);
+    lane = (uint32_t)__builtin_amdgcn_readfirstlane((int)lane);
     asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(val), "{m0}"(lane));
     return v;
 }
@@ -452,7 +456,7 @@ __device__ __forceinline__ uint32_t sc_sym(const Tables& t, uint32_t si, uint32_
 // with this pass's new significance, later ones without.  The new
 // significance of a stripe is the least fixpoint of that rule, found by
 // iterating lane-parallel; each round moves information one column east.
-__device__ uint32_t form_spp(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
+__device__ __forceinline__ uint32_t form_spp(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
                              uint32_t& newsig) {
     const bool valid = (int)threadIdx.x < w;
     const int nstripes = (h + 3) >> 2;
@@ -512,7 +516,7 @@ __device__ uint32_t form_spp(const Tables& t, Col& st, uint64_t bits, uint64_t n
 // Magnitude refinement pass, context formation (t1.rs:914-965): members are
 // significant samples not coded in this plane's SP pass; the context is 16
 // once refined, else 15/14 by any significant 8-neighbour.
-__device__ uint32_t form_mrp(Col& st, uint64_t bits, int w, int h, uint8_t* symb) {
+__device__ __forceinline__ uint32_t form_mrp(Col& st, uint64_t bits, int w, int h, uint8_t* symb) {
     const bool valid = (int)threadIdx.x < w;
     const int nstripes = (h + 3) >> 2;
     uint32_t total = 0;
@@ -544,7 +548,7 @@ __device__ uint32_t form_mrp(Col& st, uint64_t bits, int w, int h, uint8_t* symb
 // bit & todo.  Full stripes that are clean (flags.rs:202-215) and have no
 // new significance in the west column use run mode: AGG, then if any bit is
 // set the run length as two UNI decisions and the first sample's sign.
-__device__ uint32_t form_cup(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
+__device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
                              uint32_t& newsig) {
     const bool valid = (int)threadIdx.x < w;
     const int nstripes = (h + 3) >> 2;
@@ -742,21 +746,14 @@ __device__ __forceinline__ int32_t quantize(float v, float step) {
     return (int32_t)q;
 }
 
-// Tier-1 encoder: one wavefront per code-block (blockIdx.x = job index).
-// Outputs the terminated segment into out + job.out_off, the per-pass byte
-// snapshots and newly-significant counts, and the block statistics of
-// t1.rs:215-258.
-__global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
-                                                       uint8_t* __restrict__ out, BlockResult* __restrict__ results,
-                                                       uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs) {
-    __shared__ uint32_t symw[kSymCap / 4];
-    uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
+// Load one code-block, quantise it (9/7) and form the block statistics of
+// t1.rs:215-258: lane x gets column x as row magnitudes and a negative mask.
+// Returns mbp (0 for an all-zero block).
+__device__ __forceinline__ int load_block(const BlockJob& J, uint32_t (&mag)[64], uint64_t& neg, BlockResult& R) {
     const int lane = threadIdx.x;
-    const BlockJob J = jobs[blockIdx.x];
     const int w = J.w, h = J.h;
     const bool lossy = J.step > 0.0f;
-    uint32_t mag[64];
-    uint64_t neg = 0;
+    neg = 0;
     uint32_t mx = 0, nz = 0;
 #pragma unroll
     for (int y = 0; y < 64; y++) {
@@ -776,7 +773,6 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     mx = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_max(mx));
     nz = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sum(nz));
     const int mbp = mx ? 32 - __builtin_clz(mx) : 0;
-    BlockResult R;
     R.max_mag = mx;
     R.nnz = (uint16_t)nz;
     R.mbp = (uint8_t)mbp;
@@ -787,6 +783,24 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     R.pad = 0;
     R.nbytes = 0;
     R.npasses = 0;
+    return mbp;
+}
+
+// Tier-1 encoder: one wavefront per code-block (blockIdx.x = job index).
+// Outputs the terminated segment into out + job.out_off, the per-pass byte
+// snapshots and newly-significant counts, and the block statistics.
+__global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+                                                       uint8_t* __restrict__ out, BlockResult* __restrict__ results,
+                                                       uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs) {
+    __shared__ uint32_t symw[kSymCap / 4];
+    uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
+    const int lane = threadIdx.x;
+    const BlockJob J = jobs[blockIdx.x];
+    const int w = J.w, h = J.h;
+    uint32_t mag[64];
+    uint64_t neg;
+    BlockResult R;
+    const int mbp = load_block(J, mag, neg, R);
     if (mbp == 0) {
         if (lane == 0) results[blockIdx.x] = R;
         return;
@@ -862,6 +876,274 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     if (lane == 0) results[blockIdx.x] = R;
 }
 
+// ---- batched Tier-1 encoder: context formation, then one lane per block ------
+//
+// For large batches the MQ coder runs one code-block per lane: a VALU
+// instruction then advances 64 coders, where the wave-per-block coder above
+// spends one scalar instruction on one.  t1_form_kernel writes each block's
+// (context, decision) bytes, all passes in order, into one contiguous
+// stream; t1_mq_kernel codes the streams.  Every active lane consumes one
+// symbol per step, so the lanes of a wave stay in step and their 16-byte
+// stream reads are issued together.
+
+// Stream region of a block with mbp bit-planes: per plane at most one
+// symbol per sample plus two extra per run-mode stripe-column (AGG=1 adds
+// AGG and two UNI and drops the samples before the run length), plus one
+// sign per sample, plus the reader's 32-byte look-ahead; 16-byte aligned.
+__host__ __device__ __forceinline__ uint64_t stream_bound(uint32_t mbp) {
+    return ((uint64_t)mbp * 6144u + 4096u + 32u + 15u) & ~(uint64_t)15u;
+}
+
+// Context formation for the lane-per-block coder: same passes as the fused
+// kernel, symbols copied from LDS to the block's stream in global memory.
+// pend[b * kMaxPasses + k] = symbol count up to the end of pass k.
+__global__ __launch_bounds__(64) void t1_form_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+                                                     BlockResult* __restrict__ results, uint32_t* __restrict__ pend,
+                                                     uint16_t* __restrict__ newsigs, uint32_t* __restrict__ soff16,
+                                                     uint8_t* __restrict__ pool, unsigned long long* __restrict__ pool_used,
+                                                     unsigned long long pool_cap) {
+    __shared__ uint32_t symw[kSymCap / 4];
+    uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
+    const int lane = threadIdx.x;
+    const BlockJob J = jobs[blockIdx.x];
+    const int w = J.w, h = J.h;
+    uint32_t mag[64];
+    uint64_t neg;
+    BlockResult R;
+    const int mbp = load_block(J, mag, neg, R);
+    if (mbp == 0) {
+        if (lane == 0) {
+            results[blockIdx.x] = R;
+            soff16[blockIdx.x] = 0;
+        }
+        return;
+    }
+    const uint64_t bound = stream_bound((uint32_t)mbp);
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(pool_used, (unsigned long long)bound);
+    base = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(base >> 32), 0) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, 0);
+    bool fits = base + bound <= pool_cap;
+    uint8_t* dst = pool + base;
+    Tables t;
+    load_tables(t, tabs, J.band);
+    Col st = {0, 0, 0, 0};
+    uint32_t end0 = 0, end1 = 0, ns0 = 0, ns1 = 0;
+    uint32_t np = 0, cum = 0;
+    auto emit_pass = [&](uint32_t n, uint32_t ns) {
+        __syncthreads();
+        fits = fits && cum + n + 32u <= bound;  // never past the region, whatever the bound's margin
+        if (fits)
+            for (uint32_t i = lane; i < n; i += 64u) dst[cum + i] = symb[i];
+        cum += n;
+        if (np < 64u) {
+            end0 = wlane(end0, cum, np);
+            ns0 = wlane(ns0, ns, np);
+        } else {
+            end1 = wlane(end1, cum, np - 64u);
+            ns1 = wlane(ns1, ns, np - 64u);
+        }
+        np++;
+        __syncthreads();
+    };
+    const int top = mbp - 1;
+    for (int p = top; p >= 0; p--) {
+        uint64_t bits = 0;
+#pragma unroll
+        for (int y = 0; y < 64; y++) bits |= (uint64_t)((mag[y] >> p) & 1u) << y;
+        uint32_t ns, n;
+        if (p != top) {
+            n = form_spp(t, st, bits, neg, w, h, symb, ns);
+            emit_pass(n, ns);
+            n = form_mrp(st, bits, w, h, symb);
+            emit_pass(n, 0);
+        }
+        n = form_cup(t, st, bits, neg, w, h, symb, ns);
+        emit_pass(n, ns);
+        st.vis = 0;
+    }
+    R.npasses = (uint8_t)np;
+    R.overflow = fits ? 0 : 2;
+    uint32_t* pe = pend + (size_t)blockIdx.x * kMaxPasses;
+    uint16_t* nsp = newsigs + (size_t)blockIdx.x * kMaxPasses;
+    if ((uint32_t)lane < np) {
+        pe[lane] = end0;
+        nsp[lane] = (uint16_t)ns0;
+    }
+    if ((uint32_t)lane + 64u < np && lane + 64 < kMaxPasses) {
+        pe[lane + 64] = end1;
+        nsp[lane + 64] = (uint16_t)ns1;
+    }
+    if (lane == 0) {
+        results[blockIdx.x] = R;
+        soff16[blockIdx.x] = (uint32_t)(base >> 4);
+    }
+}
+
+namespace {
+// One lane's MQ encoder state (mq/mod.rs:163-432); context entries live in
+// LDS, cst[ctx * 64 + lane], in the table-entry form used by MqE.
+struct LaneMq {
+    uint32_t a, c, ct, pos, last, ovf;
+    uint8_t* o;
+    uint32_t cap;
+};
+__device__ __forceinline__ void lmq_emit(LaneMq& m, uint32_t b) {
+    if (m.pos > 0) {
+        if (m.pos - 1u < m.cap) m.o[m.pos - 1u] = (uint8_t)m.last;
+        else m.ovf = 1u;
+    }
+    m.pos++;
+    m.last = b & 0xffu;
+}
+__device__ __forceinline__ void lmq_byteout(LaneMq& m) {
+    if (m.last == 0xffu) {
+        lmq_emit(m, m.c >> 20);
+        m.c &= 0xfffffu;
+        m.ct = 7;
+    } else if ((m.c & 0x8000000u) == 0) {
+        lmq_emit(m, m.c >> 19);
+        m.c &= 0x7ffffu;
+        m.ct = 8;
+    } else {
+        m.last = (m.last + 1u) & 0xffu;
+        if (m.last == 0xffu) {
+            m.c &= 0x7ffffffu;
+            lmq_emit(m, m.c >> 20);
+            m.c &= 0xfffffu;
+            m.ct = 7;
+        } else {
+            lmq_emit(m, m.c >> 19);
+            m.c &= 0x7ffffu;
+            m.ct = 8;
+        }
+    }
+}
+}  // namespace
+
+// MQ coding of the streams, one code-block per lane (block = blockIdx.x * 64
+// + lane).  Writes the segment bytes, the per-pass byte snapshots and the
+// final length into the block's BlockResult.
+//
+// The step counter is wave-uniform: lane b codes symbol s of its stream at
+// step s while s < its stream length.  Each step is branch-free apart from
+// the byte output: the MPS-without-renormalisation case is the exchange
+// formula with the state written back unchanged and a zero shift.
+__global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __restrict__ jobs,
+                                                   const uint32_t* __restrict__ tabs, const uint8_t* __restrict__ pool,
+                                                   const uint32_t* __restrict__ soff16, const uint32_t* __restrict__ pend,
+                                                   BlockResult* __restrict__ results, uint32_t* __restrict__ snaps,
+                                                   uint8_t* __restrict__ out) {
+    __shared__ uint32_t cst[32 * 64];  // 19 contexts used; sized for any 5-bit context code
+    __shared__ uint32_t tab[64];
+    const int lane = threadIdx.x;
+    tab[lane] = tabs[lane];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 19; k++)
+        cst[k * 64 + lane] = tab[k == CTX_UNI ? 46 : k == CTX_AGG ? 3 : k == CTX_ZC ? 4 : 0];
+    const int b = blockIdx.x * 64 + lane;
+    uint32_t np = 0, total = 0;
+    if (b < nb && results[b].overflow == 0) np = results[b].npasses;
+    const uint32_t* pe = pend + (size_t)b * kMaxPasses;
+    if (np) total = pe[np - 1];
+    const uint32_t smax = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_max(total));
+    if (smax == 0) return;
+    uint32_t* cx = cst + lane;
+    const uint4* src = reinterpret_cast<const uint4*>(pool + (np ? (uint64_t)soff16[b] << 4 : 0));
+    uint32_t* sn = snaps + (size_t)b * kMaxPasses;
+    LaneMq m;
+    m.a = 0x8000u;
+    m.c = 0;
+    m.ct = 12;
+    m.pos = 0;
+    m.last = 0;
+    m.ovf = 0;
+    m.o = nullptr;
+    m.cap = 0;
+    if (np) {
+        const BlockJob& J = jobs[b];
+        m.o = out + J.out_off;
+        m.cap = J.out_cap;
+    }
+    uint32_t k = 0, kend = np ? pe[0] : 0xffffffffu;
+    uint4 cur = make_uint4(0, 0, 0, 0), nxt = cur;
+    if (np) {
+        cur = src[0];
+        nxt = src[1];
+    }
+    auto step = [&](uint32_t si, uint32_t sym) {
+        // passes that end before symbol si (passes may be empty)
+        while (__builtin_amdgcn_ballot_w64(si == kend)) {
+            if (si == kend) {
+                sn[k] = m.pos;
+                k++;
+                kend = k < np ? pe[k] : 0xffffffffu;
+            }
+        }
+        if (si < total) {
+            uint32_t* cp = cx + (sym & 31u) * 64u;
+            const uint32_t e = *cp;
+            const uint32_t qe = e & 0xffffu;
+            const uint32_t a2 = m.a - qe;
+            const uint32_t x = e ^ (sym << 24);  // bit 31: decision differs from the MPS
+            const bool lps = (int32_t)x < 0;
+            const bool keep = !lps && (a2 & 0x8000u);
+            const bool to_qe = (a2 < qe) != lps;
+            m.a = to_qe ? qe : a2;
+            m.c += to_qe ? 0u : qe;
+            const uint32_t next = (e >> (lps ? 22 : 16)) & 63u;
+            const uint32_t mps = (lps ? e ^ (e << 3) : e) & 0x80000000u;
+            const uint32_t ne = tab[next] | mps;
+            *cp = keep ? e : ne;
+            uint32_t sh = (uint32_t)__builtin_clz(m.a) - 16u;
+            while (sh >= m.ct) {
+                m.a <<= m.ct;
+                m.c <<= m.ct;
+                sh -= m.ct;
+                lmq_byteout(m);
+            }
+            m.a <<= sh;
+            m.c <<= sh;
+            m.ct -= sh;
+        }
+    };
+    for (uint32_t s = 0; s < smax; s += 16u) {
+        if (s) {
+            cur = nxt;
+            // look-ahead only within this lane's region: its stream plus the
+            // 32 bytes of slack the region bound provides
+            if (np && (s >> 4) + 1u <= (total + 15u) >> 4) nxt = src[(s >> 4) + 1];
+        }
+        const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t wd = w4[q];
+            step(s + 4 * q, wd);
+            step(s + 4 * q + 1, wd >> 8);
+            step(s + 4 * q + 2, wd >> 16);
+            step(s + 4 * q + 3, wd >> 24);
+        }
+    }
+    if (!np) return;
+    for (; k < np; k++) sn[k] = m.pos;  // passes ending at the stream's end
+    // flush + take_bytes (mq/mod.rs:244-250, :373-379, :416-423)
+    const uint32_t temp = m.c + m.a;
+    m.c |= 0xffffu;
+    if (m.c >= temp) m.c -= 0x8000u;
+    m.c <<= m.ct;
+    lmq_byteout(m);
+    m.c <<= m.ct;
+    lmq_byteout(m);
+    const uint32_t fin = m.last == 0xffu ? m.pos - 1u : m.pos;
+    if (m.pos > 0) {
+        if (m.pos - 1u < m.cap) m.o[m.pos - 1u] = (uint8_t)m.last;
+        else m.ovf = 1u;
+    }
+    results[b].nbytes = fin;
+    results[b].overflow = m.ovf ? 1 : 0;
+}
+
 // Tier-1 decoder: one wavefront per code-block.  Decodes `passes` coding
 // passes and writes the reconstructed coefficients into the plane:
 // dequantised f32 (sign * (|q| + 0.5)) * step for 9/7 (scalar.rs:233-240),
@@ -931,6 +1213,16 @@ void launch_t1_encode(const BlockJob* jobs, int n, const uint32_t* tabs, uint8_t
                       uint16_t* newsigs, hipStream_t st) {
     if (n > 0) t1_encode_kernel<<<n, 64, 0, st>>>(jobs, tabs, out, res, snaps, newsigs);
 }
+void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResult* res, uint32_t* pend, uint16_t* newsigs,
+                    uint32_t* soff16, uint8_t* pool, unsigned long long* pool_used, unsigned long long pool_cap,
+                    hipStream_t st) {
+    if (n > 0) t1_form_kernel<<<n, 64, 0, st>>>(jobs, tabs, res, pend, newsigs, soff16, pool, pool_used, pool_cap);
+}
+void launch_t1_mq(const BlockJob* jobs, int n, const uint32_t* tabs, const uint8_t* pool, const uint32_t* soff16,
+                  const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, hipStream_t st) {
+    if (n > 0) t1_mq_kernel<<<(n + 63) / 64, 64, 0, st>>>(n, jobs, tabs, pool, soff16, pend, res, snaps, out);
+}
+uint64_t t1_stream_bound(uint32_t mbp) { return stream_bound(mbp); }
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st) {
     if (n > 0) t1_decode_kernel<<<n, 64, 0, st>>>(jobs, tabs, payload);
 }

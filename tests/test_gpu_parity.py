@@ -288,3 +288,64 @@ def test_crate_api_roundtrip(gpu):
     with pytest.raises(j.InvalidInput, match="outside"):
         bad = j.Image(2, 2, [j.Component(np.array([0, 1, 2, 300], np.int32), 2, 2)])
         j.encode(bad)
+
+
+# ---- lane-per-block Tier-1 coder (batched path) ------------------------------
+# The coder choice must not change a byte: every case above is repeated
+# with the lane-per-block coder forced on.
+
+@pytest.fixture
+def lane_mode(lib):
+    lib.jp2g_set_t1_mode(2, 0)
+    yield lib
+    lib.jp2g_set_t1_mode(0, 4000)
+
+
+@pytest.mark.parametrize("name,make,qs", CASES, ids=[c[0] for c in CASES])
+def test_lane_coder_codestream_identical(gpu, ora, lane_mode, name, make, qs):
+    px = make()
+    for q in qs:
+        ours = j.encode_u8(px, q)
+        ref = ora.encode(px, q)
+        assert ours == ref, f"{name} q{q}: {len(ours)} vs {len(ref)} bytes"
+    assert lane_mode.jp2g_last_t1_lanes() == 1
+
+
+def test_lane_coder_block_records(gpu, ora, lane_mode):
+    px = synth.document_page(700, 500, seed=8)
+    lane_mode.jp2g_set_debug(1)
+    try:
+        ours = j.encode_u8(px, 85)
+        data, enc = ora.encode(px, 85, keep=True)
+        assert ours == data
+        for i in range(lane_mode.jp2g_last_block_count()):
+            gi = np.zeros(16, np.int32)
+            lane_mode.jp2g_last_block_info(i, gi.ctypes.data)
+            assert gi.tolist() == enc.info(i).tolist(), i
+            npass = int(gi[11])
+            gp = np.zeros(5 * max(npass, 1), np.int32)
+            lane_mode.jp2g_last_block_passes(i, gp.ctypes.data)
+            assert np.array_equal(gp[:5 * npass].reshape(npass, 5), enc.passes(i)), i
+    finally:
+        lane_mode.jp2g_set_debug(0)
+
+
+def test_lane_coder_full_page(gpu, ora, lane_mode):
+    px = synth.document_page()
+    assert j.encode_u8(px, 85) == ora.encode(px, 85)
+
+
+def test_auto_mode_batch_crosses_threshold(gpu, ora, lib):
+    """A batch above the block threshold runs the lane coder, one below it
+    the wave coder; both give the single-image bytes."""
+    pages = list(synth.mixed_pages(5, scale=0.12))
+    singles = [ora.encode(px, q) for px, q in pages]
+    lib.jp2g_set_t1_mode(0, 50)
+    try:
+        assert j.encode_many(pages) == singles
+        assert lib.jp2g_last_t1_lanes() == 1
+        lib.jp2g_set_t1_mode(0, 10 ** 9)
+        assert j.encode_many(pages) == singles
+        assert lib.jp2g_last_t1_lanes() == 0
+    finally:
+        lib.jp2g_set_t1_mode(0, 4000)
