@@ -889,14 +889,19 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
 // Stream region of a block with mbp bit-planes: per plane at most one
 // symbol per sample plus two extra per run-mode stripe-column (AGG=1 adds
 // AGG and two UNI and drops the samples before the run length), plus one
-// sign per sample, plus the reader's 32-byte look-ahead; 16-byte aligned.
+// sign per sample, one pass marker per pass, and the reader's 32-byte
+// look-ahead; 16-byte aligned.
 __host__ __device__ __forceinline__ uint64_t stream_bound(uint32_t mbp) {
-    return ((uint64_t)mbp * 6144u + 4096u + 32u + 15u) & ~(uint64_t)15u;
+    return ((uint64_t)mbp * 6144u + 4096u + kMaxPasses + 32u + 15u) & ~(uint64_t)15u;
 }
+// Pass marker: context 31, whose entry (Qe = 0, MPS 0) makes coding it a
+// no-op (a stays >= 0x8000, c += 0, no renormalisation).
+constexpr uint32_t kMarker = 31u;
 
 // Context formation for the lane-per-block coder: same passes as the fused
-// kernel, symbols copied from LDS to the block's stream in global memory.
-// pend[b * kMaxPasses + k] = symbol count up to the end of pass k.
+// kernel, symbols copied from LDS to the block's stream in global memory,
+// each pass followed by a marker.  pend[b * kMaxPasses + k] = stream length
+// up to and including pass k's marker.
 __global__ __launch_bounds__(64) void t1_form_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                      BlockResult* __restrict__ results, uint32_t* __restrict__ pend,
                                                      uint16_t* __restrict__ newsigs, uint32_t* __restrict__ soff16,
@@ -932,10 +937,12 @@ __global__ __launch_bounds__(64) void t1_form_kernel(const BlockJob* __restrict_
     uint32_t np = 0, cum = 0;
     auto emit_pass = [&](uint32_t n, uint32_t ns) {
         __syncthreads();
-        fits = fits && cum + n + 32u <= bound;  // never past the region, whatever the bound's margin
-        if (fits)
+        fits = fits && cum + n + 1u + 32u <= bound;  // never past the region, whatever the bound's margin
+        if (fits) {
             for (uint32_t i = lane; i < n; i += 64u) dst[cum + i] = symb[i];
-        cum += n;
+            if (lane == 0) dst[cum + n] = (uint8_t)kMarker;
+        }
+        cum += n + 1u;
         if (np < 64u) {
             end0 = wlane(end0, cum, np);
             ns0 = wlane(ns0, ns, np);
@@ -988,36 +995,24 @@ struct LaneMq {
     uint8_t* o;
     uint32_t cap;
 };
-__device__ __forceinline__ void lmq_emit(LaneMq& m, uint32_t b) {
+// Byte-out (mq/mod.rs:373-414) as selects: the carry into the held byte,
+// the 7-bit byte after 0xFF, then commit the held byte and hold the new one.
+__device__ __forceinline__ void lmq_byteout(LaneMq& m) {
+    const bool lff = m.last == 0xffu;
+    const bool carry = !lff && (m.c & 0x8000000u);
+    const uint32_t held = carry ? (m.last + 1u) & 0xffu : m.last;
+    const bool clr = carry && held == 0xffu;
+    const uint32_t c1 = clr ? m.c & 0x7ffffffu : m.c;
+    const bool seven = lff || clr;
+    const uint32_t byte = seven ? c1 >> 20 : c1 >> 19;
+    m.c = c1 & (seven ? 0xfffffu : 0x7ffffu);
+    m.ct = seven ? 7u : 8u;
     if (m.pos > 0) {
-        if (m.pos - 1u < m.cap) m.o[m.pos - 1u] = (uint8_t)m.last;
+        if (m.pos - 1u < m.cap) m.o[m.pos - 1u] = (uint8_t)held;
         else m.ovf = 1u;
     }
     m.pos++;
-    m.last = b & 0xffu;
-}
-__device__ __forceinline__ void lmq_byteout(LaneMq& m) {
-    if (m.last == 0xffu) {
-        lmq_emit(m, m.c >> 20);
-        m.c &= 0xfffffu;
-        m.ct = 7;
-    } else if ((m.c & 0x8000000u) == 0) {
-        lmq_emit(m, m.c >> 19);
-        m.c &= 0x7ffffu;
-        m.ct = 8;
-    } else {
-        m.last = (m.last + 1u) & 0xffu;
-        if (m.last == 0xffu) {
-            m.c &= 0x7ffffffu;
-            lmq_emit(m, m.c >> 20);
-            m.c &= 0xfffffu;
-            m.ct = 7;
-        } else {
-            lmq_emit(m, m.c >> 19);
-            m.c &= 0x7ffffu;
-            m.ct = 8;
-        }
-    }
+    m.last = byte & 0xffu;
 }
 }  // namespace
 
@@ -1066,47 +1061,41 @@ __global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __res
         m.o = out + J.out_off;
         m.cap = J.out_cap;
     }
-    uint32_t k = 0, kend = np ? pe[0] : 0xffffffffu;
+    uint32_t k = 0;
     uint4 cur = make_uint4(0, 0, 0, 0), nxt = cur;
     if (np) {
         cur = src[0];
         nxt = src[1];
     }
+    // cst row kMarker stays 0: Qe 0, MPS 0
+    cst[kMarker * 64 + lane] = 0;
     auto step = [&](uint32_t si, uint32_t sym) {
-        // passes that end before symbol si (passes may be empty)
-        while (__builtin_amdgcn_ballot_w64(si == kend)) {
-            if (si == kend) {
-                sn[k] = m.pos;
-                k++;
-                kend = k < np ? pe[k] : 0xffffffffu;
-            }
+        sym = si < total ? sym & 0xffu : kMarker;  // past the end: no-op symbols
+        if (si < total && (sym & 31u) == kMarker) sn[k++] = m.pos;  // end of a pass
+        uint32_t* cp = cx + (sym & 31u) * 64u;
+        const uint32_t e = *cp;
+        const uint32_t qe = e & 0xffffu;
+        const uint32_t a2 = m.a - qe;
+        const uint32_t x = e ^ (sym << 24);  // bit 31: decision differs from the MPS
+        const bool lps = (int32_t)x < 0;
+        const bool keep = !lps && (a2 & 0x8000u);
+        const bool to_qe = (a2 < qe) != lps;
+        m.a = to_qe ? qe : a2;
+        m.c += to_qe ? 0u : qe;
+        const uint32_t next = (e >> (lps ? 22 : 16)) & 63u;
+        const uint32_t mps = (lps ? e ^ (e << 3) : e) & 0x80000000u;
+        const uint32_t ne = tab[next] | mps;
+        *cp = keep ? e : ne;
+        uint32_t sh = (uint32_t)__builtin_clz(m.a) - 16u;
+        while (sh >= m.ct) {
+            m.a <<= m.ct;
+            m.c <<= m.ct;
+            sh -= m.ct;
+            lmq_byteout(m);
         }
-        if (si < total) {
-            uint32_t* cp = cx + (sym & 31u) * 64u;
-            const uint32_t e = *cp;
-            const uint32_t qe = e & 0xffffu;
-            const uint32_t a2 = m.a - qe;
-            const uint32_t x = e ^ (sym << 24);  // bit 31: decision differs from the MPS
-            const bool lps = (int32_t)x < 0;
-            const bool keep = !lps && (a2 & 0x8000u);
-            const bool to_qe = (a2 < qe) != lps;
-            m.a = to_qe ? qe : a2;
-            m.c += to_qe ? 0u : qe;
-            const uint32_t next = (e >> (lps ? 22 : 16)) & 63u;
-            const uint32_t mps = (lps ? e ^ (e << 3) : e) & 0x80000000u;
-            const uint32_t ne = tab[next] | mps;
-            *cp = keep ? e : ne;
-            uint32_t sh = (uint32_t)__builtin_clz(m.a) - 16u;
-            while (sh >= m.ct) {
-                m.a <<= m.ct;
-                m.c <<= m.ct;
-                sh -= m.ct;
-                lmq_byteout(m);
-            }
-            m.a <<= sh;
-            m.c <<= sh;
-            m.ct -= sh;
-        }
+        m.a <<= sh;
+        m.c <<= sh;
+        m.ct -= sh;
     };
     for (uint32_t s = 0; s < smax; s += 16u) {
         if (s) {
@@ -1126,7 +1115,6 @@ __global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __res
         }
     }
     if (!np) return;
-    for (; k < np; k++) sn[k] = m.pos;  // passes ending at the stream's end
     // flush + take_bytes (mq/mod.rs:244-250, :373-379, :416-423)
     const uint32_t temp = m.c + m.a;
     m.c |= 0xffffu;
