@@ -33,10 +33,17 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the post-run oracle check")
     ap.add_argument("--batch-pages", type=int, default=8,
                     help="pages in the batched-encode measurement (0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="length of the serving leg (Poisson arrivals of page encodes); 0 = skip")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="serving arrival rate, requests/s")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def enc_plain_host(j, px, q):
+    return j.encode_u8(px, q)
 
 
 def level_areas(w, h, levels):
@@ -163,6 +170,49 @@ def main():
                                 "host_tier2": round(tb[3], 3)}}
         del pages
 
+    # ---- serving: Poisson arrivals of page encodes, continuous batching ------
+    serve = None
+    if args.serve_seconds > 0:
+        import threading
+
+        pool = [synth.document_page(W, H, seed=2 + 1000 * k) for k in range(4)]
+        want = [enc_plain_host(j, p, Q) for p in pool]
+        n = max(1, int(args.serve_rate * args.serve_seconds))
+        rng = np.random.default_rng(5)
+        arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+        ids = [None] * n
+        done = [None] * n
+        with j.Server(slots=2, mem_budget_mib=32768, max_batch=16) as srv:
+            for p in pool:  # warm the two engines' arenas
+                srv.wait(srv.submit_encode(p, Q))
+
+            def waiter():
+                for i in range(n):
+                    while ids[i] is None:
+                        time.sleep(0.0002)
+                    done[i] = srv.wait(ids[i])
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t0 = time.perf_counter()
+            for i in range(n):
+                d = t0 + arrivals[i] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = srv.submit_encode(pool[i % len(pool)], Q)
+            th.join()
+            t_end = time.perf_counter()
+            st = srv.stats()
+        lat = np.array([r.total_ms for r in done])
+        ok = all(done[i].value == want[i % len(pool)] for i in range(n))
+        serve = {"workload": f"{W}x{H} gray q{Q} page encodes, host pixels, Poisson arrivals",
+                 "rate_req_s": args.serve_rate, "requests": n, "slots": 2,
+                 "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                 "max_ms": round(float(lat.max()), 2),
+                 "throughput_pages_s": round(n / (t_end - t0), 1),
+                 "mean_batch": round(float(np.mean([r.batch_size for r in done])), 2),
+                 "batches": int(st["batches"]), "results_equal_alone": bool(ok)}
+
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib << 20
         a = torch.ones(n, dtype=torch.uint8, device=dev)
@@ -225,6 +275,7 @@ def main():
         "stage_ms": stage_ms,
         "parity_vs_oracle": check,
         "batched_encode": batched,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 

@@ -145,6 +145,41 @@ typedef struct {
 jp2lam_status jp2lam_encode_many(jp2lam_encode_item* items, size_t n);
 jp2lam_status jp2lam_decode_many(jp2lam_decode_item* items, size_t n);
 
+/* ---- serving (no counterpart in the reference) ---------------------------
+ * Continuous batching over `slots` device engines: a free slot takes the
+ * oldest request and every later one that fits its share of the memory
+ * budget (at most `max_batch`), runs them as one batch, and publishes the
+ * results; batches on different slots overlap on the GPU.  Each result is
+ * the same bytes/pixels as the request alone.  Host pixels and streams are
+ * copied at submission; device pixels (on_device) must stay valid and
+ * complete until the request is done. */
+typedef struct jp2lam_server jp2lam_server;
+typedef struct {
+    int32_t status;        /* the request's own status */
+    uint8_t* data;         /* encode: codestream; decode: interleaved u8; free with jp2lam_free */
+    size_t len;
+    uint32_t width, height, channels, batch_size;
+    double queue_ms, run_ms, total_ms;  /* submit->start, start->done, submit->done */
+} jp2lam_served;
+jp2lam_server* jp2lam_server_new(int slots, size_t mem_budget_mib, int max_batch);
+/* Return a request id, or 0 with jp2lam_last_error() set. */
+uint64_t jp2lam_server_submit_encode(jp2lam_server* srv, const uint8_t* pixels, uint32_t width, uint32_t height,
+                                     uint32_t channels, uint8_t quality, uint8_t format, uint8_t on_device);
+uint64_t jp2lam_server_submit_decode(jp2lam_server* srv, const uint8_t* bytes, size_t len);
+/* 1: done (out filled), 0: timed out (timeout_ms >= 0), -1: bad arguments. */
+int jp2lam_server_wait(jp2lam_server* srv, uint64_t id, int timeout_ms, jp2lam_served* out);
+/* {submitted, completed, batches, largest batch} */
+void jp2lam_server_stats(jp2lam_server* srv, uint64_t* out4);
+void jp2lam_server_free(jp2lam_server* srv);
+
+/* ---- device buffers (no counterpart in the reference) ----------------------
+ * For callers without their own HIP allocator: HBM buffers to hold pixels
+ * for on_device encode items and decode targets.  kind: 0 host->device,
+ * 1 device->host, 2 device->device; copies are synchronous. */
+void* jp2lam_device_alloc(size_t bytes);
+jp2lam_status jp2lam_device_copy(void* dst, const void* src, size_t bytes, int kind);
+void jp2lam_device_free(void* ptr);
+
 /* ---- stage-level entry points for parity tests ------------------------------- */
 /* Forward colour + DWT: coefficient planes (Mallat layout) as the Tier-1
  * stage sees them before quantisation: f32 (quality < 100) or i32.  `out`

@@ -27,6 +27,7 @@ __all__ = [
     "Jp2LamError", "InvalidInput", "EncodeFailed", "DecodeFailed", "UnsupportedFeature", "IoError",
     "encode", "encode_to_writer", "decode_jp2", "decode_from_reader", "inspect_jp2", "BatchEncoder",
     "BatchDecoder", "encode_batch", "decode_batch", "encode_u8", "decode_u8", "encode_many", "decode_many",
+    "Server", "Served",
 ]
 
 
@@ -423,3 +424,98 @@ def decode_many(streams: Sequence, device_outputs: Optional[Sequence] = None):
             a = outs[i]
             res.append(a.reshape(it.height, it.width) if it.channels == 1 else a.reshape(it.height, it.width, it.channels))
     return res
+
+
+# ---- serving: continuous batching (no counterpart in the reference) ----------
+class Served:
+    """A finished request: ``value`` is the codestream (encode) or an
+    ``(H, W[, C])`` uint8 array (decode), or the :class:`Jp2LamError`.
+    Times are milliseconds: queue (submit->start), run (start->done),
+    total (submit->done); ``batch_size`` is the batch it ran in."""
+
+    __slots__ = ("value", "queue_ms", "run_ms", "total_ms", "batch_size")
+
+    def __init__(self, value, queue_ms, run_ms, total_ms, batch_size):
+        self.value, self.queue_ms, self.run_ms = value, queue_ms, run_ms
+        self.total_ms, self.batch_size = total_ms, batch_size
+
+
+class Server:
+    """Request loop with continuous batching over ``slots`` device engines.
+
+    ``submit_encode``/``submit_decode`` return an id at once; ``wait(id)``
+    returns a :class:`Served`.  Requests that arrive while batches run are
+    admitted as a slot frees, up to ``mem_budget_mib`` of device memory per
+    server and ``max_batch`` requests per batch."""
+
+    def __init__(self, slots: int = 2, mem_budget_mib: int = 16384, max_batch: int = 16):
+        self._lib = _lib.load()
+        self._h = self._lib.jp2lam_server_new(slots, mem_budget_mib, max_batch)
+        self._kinds = {}
+        if not self._h:
+            raise IoError(self._lib.jp2lam_last_error().decode())
+
+    def submit_encode(self, pixels, quality: int = 85, format: OutputFormat = OutputFormat.JP2,
+                      device=None) -> int:
+        """``pixels``: (H, W) or (H, W, 3) uint8 array, or with ``device``
+        = ``(device_pointer, H, W, C)`` for pixels already in HBM."""
+        if device is not None:
+            ptr, h, w, ch = device
+            rid = self._lib.jp2lam_server_submit_encode(self._h, ptr, w, h, ch, min(int(quality), 255), int(format), 1)
+        else:
+            a = np.ascontiguousarray(pixels, dtype=np.uint8)
+            ch = 1 if a.ndim == 2 else a.shape[2]
+            rid = self._lib.jp2lam_server_submit_encode(self._h, a.ctypes.data, a.shape[1], a.shape[0], ch,
+                                                        min(int(quality), 255), int(format), 0)
+        if not rid:
+            raise InvalidInput(self._lib.jp2lam_last_error().decode())
+        self._kinds[rid] = "encode"
+        return rid
+
+    def submit_decode(self, data) -> int:
+        b = _as_buffer(data)
+        rid = self._lib.jp2lam_server_submit_decode(self._h, C.cast(C.c_char_p(b), C.c_void_p), len(b))
+        if not rid:
+            raise InvalidInput(self._lib.jp2lam_last_error().decode())
+        self._kinds[rid] = "decode"
+        return rid
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> Optional[Served]:
+        out = _lib.ServedC()
+        rc = self._lib.jp2lam_server_wait(self._h, rid, timeout_ms, C.byref(out))
+        if rc < 0:
+            raise InvalidInput(self._lib.jp2lam_last_error().decode())
+        if rc == 0:
+            return None
+        kind = self._kinds.pop(rid, "encode")
+        if out.status != 0:
+            value = _ERRORS.get(out.status, IoError)(self._lib.jp2lam_last_error().decode())
+        elif kind == "encode":
+            value = _take_bytes(out.data, out.len) if out.len else b""
+        else:
+            arr = np.frombuffer(_take_bytes(out.data, out.len), dtype=np.uint8) if out.len else np.zeros(0, np.uint8)
+            value = arr.reshape(out.height, out.width) if out.channels == 1 else \
+                arr.reshape(out.height, out.width, out.channels)
+        return Served(value, out.queue_ms, out.run_ms, out.total_ms, out.batch_size)
+
+    def stats(self) -> dict:
+        v = (C.c_uint64 * 4)()
+        self._lib.jp2lam_server_stats(self._h, v)
+        return {"submitted": v[0], "completed": v[1], "batches": v[2], "largest_batch": v[3]}
+
+    def close(self):
+        if self._h:
+            self._lib.jp2lam_server_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

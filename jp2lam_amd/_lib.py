@@ -53,6 +53,12 @@ class DecodeItemC(C.Structure):
                 ("on_device", C.c_uint8), ("pad", C.c_uint8 * 3), ("status", C.c_int32)]
 
 
+class ServedC(C.Structure):
+    _fields_ = [("status", C.c_int32), ("data", u8p), ("len", C.c_size_t), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("channels", C.c_uint32), ("batch_size", C.c_uint32),
+                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double)]
+
+
 # (name, restype, argtypes) for every symbol the header declares
 SIGNATURES = [
     ("jp2lam_encode", C.c_int, [C.POINTER(ImageViewC), C.POINTER(EncodeOptionsC), C.POINTER(u8p), C.POINTER(C.c_size_t)]),
@@ -82,6 +88,16 @@ SIGNATURES = [
                                        C.c_void_p, C.c_void_p, C.c_size_t]),
     ("jp2g_t1_decode_block", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_void_p, C.c_size_t, C.c_void_p]),
+    ("jp2lam_device_alloc", C.c_void_p, [C.c_size_t]),
+    ("jp2lam_device_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    ("jp2lam_device_free", None, [C.c_void_p]),
+    ("jp2lam_server_new", C.c_void_p, [C.c_int, C.c_size_t, C.c_int]),
+    ("jp2lam_server_submit_encode", C.c_uint64, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_uint8, C.c_uint8, C.c_uint8]),
+    ("jp2lam_server_submit_decode", C.c_uint64, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("jp2lam_server_wait", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(ServedC)]),
+    ("jp2lam_server_stats", None, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("jp2lam_server_free", None, [C.c_void_p]),
     ("jp2g_set_debug", None, [C.c_int]),
     ("jp2g_set_t1_mode", None, [C.c_int, C.c_uint32]),
     ("jp2g_last_t1_lanes", C.c_int, []),

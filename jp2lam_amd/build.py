@@ -22,7 +22,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "libjp2lam_amd.so"
 BUILD_DIR = PKG / "_build"
 
-SOURCES = ["kernels_dwt.hip", "kernels_t1.hip", "kernels_pcrd.hip", "host.cpp", "engine.cpp", "api.cpp"]
+SOURCES = ["kernels_dwt.hip", "kernels_t1.hip", "kernels_pcrd.hip", "host.cpp", "engine.cpp", "server.cpp", "api.cpp"]
 ARCH = os.environ.get("JP2LAM_AMD_ARCH", "gfx950")
 
 
@@ -37,7 +37,7 @@ def _flags(src: str) -> list[str]:
     f = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
     if src.endswith(".cpp"):
-        f += ["-x", "hip"]
+        f += ["-x", "hip", "-pthread"]
     return f
 
 
@@ -58,7 +58,7 @@ def build(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

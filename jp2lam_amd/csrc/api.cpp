@@ -8,6 +8,7 @@
 #include <sstream>
 
 #include "engine.hpp"
+#include "server.hpp"
 
 using namespace j2l;
 
@@ -718,5 +719,116 @@ void jp2g_tables(uint16_t* qe, uint8_t* mps, uint8_t* nmps, uint8_t* nlps, uint8
     expanded_tables(qe, mps, nmps, nlps, zc, sc, spb);
 }
 int jp2g_device_available(void) { return device_available(nullptr) ? 1 : 0; }
+
+// ---- device buffers ------------------------------------------------------------
+void* jp2lam_device_alloc(size_t bytes) {
+    void* p = nullptr;
+    guarded([&] {
+        if (!device_available(nullptr)) return fail(IO, "no HIP device");
+        J2L_HIP(hipMalloc(&p, bytes ? bytes : 1));
+        return JP2LAM_OK;
+    });
+    return p;
+}
+jp2lam_status jp2lam_device_copy(void* dst, const void* src, size_t bytes, int kind) {
+    return guarded([&] {
+        if ((!dst || !src) && bytes) return fail(INVALID_INPUT, "null argument");
+        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                : kind == 1 ? hipMemcpyDeviceToHost
+                                : kind == 2 ? hipMemcpyDeviceToDevice
+                                            : hipMemcpyDefault;
+        if (kind < 0 || kind > 2) return fail(INVALID_INPUT, "copy kind must be 0, 1 or 2");
+        if (bytes) J2L_HIP(hipMemcpy(dst, src, bytes, k));
+        return JP2LAM_OK;
+    });
+}
+void jp2lam_device_free(void* ptr) {
+    if (ptr) (void)hipFree(ptr);
+}
+
+// ---- serving -------------------------------------------------------------------
+struct jp2lam_server {
+    std::unique_ptr<j2l::Server> s;
+};
+
+jp2lam_server* jp2lam_server_new(int slots, size_t mem_budget_mib, int max_batch) {
+    jp2lam_server* out = nullptr;
+    guarded([&] {
+        out = new jp2lam_server{std::make_unique<Server>(slots, mem_budget_mib << 20, max_batch)};
+        return JP2LAM_OK;
+    });
+    return out;
+}
+uint64_t jp2lam_server_submit_encode(jp2lam_server* srv, const uint8_t* pixels, uint32_t width, uint32_t height,
+                                     uint32_t channels, uint8_t quality, uint8_t format, uint8_t on_device) {
+    uint64_t id = 0;
+    guarded([&] {
+        if (!srv) return fail(INVALID_INPUT, "null argument");
+        id = srv->s->submit_encode(pixels, width, height, channels, quality, format, on_device != 0);
+        return JP2LAM_OK;
+    });
+    return id;
+}
+uint64_t jp2lam_server_submit_decode(jp2lam_server* srv, const uint8_t* bytes, size_t len) {
+    uint64_t id = 0;
+    guarded([&] {
+        if (!srv) return fail(INVALID_INPUT, "null argument");
+        id = srv->s->submit_decode(bytes, len);
+        return JP2LAM_OK;
+    });
+    return id;
+}
+int jp2lam_server_wait(jp2lam_server* srv, uint64_t id, int timeout_ms, jp2lam_served* out) {
+    int rc = -1;
+    guarded([&] {
+        if (!srv || !out) return fail(INVALID_INPUT, "null argument");
+        std::shared_ptr<Request> r = srv->s->wait(id, timeout_ms);
+        if (!r) {
+            rc = 0;
+            return JP2LAM_OK;
+        }
+        std::memset(out, 0, sizeof(*out));
+        using ms = std::chrono::duration<double, std::milli>;
+        out->queue_ms = ms(r->t_start - r->t_submit).count();
+        out->run_ms = ms(r->t_done - r->t_start).count();
+        out->total_ms = ms(r->t_done - r->t_submit).count();
+        out->batch_size = (uint32_t)r->batch_size;
+        if (r->is_encode) {
+            out->status = r->eout.status;
+            if (r->eout.status == OK) {
+                out->data = copy_out(r->eout.bytes);
+                out->len = r->eout.bytes.size();
+            } else {
+                fail(r->eout.status, r->eout.message);
+            }
+            out->width = r->enc.width;
+            out->height = r->enc.height;
+            out->channels = r->enc.colorspace == GRAY ? 1 : 3;
+        } else {
+            out->status = r->dout.status;
+            if (r->dout.status == OK) {
+                out->data = copy_out(r->dout.pixels);
+                out->len = r->dout.pixels.size();
+            } else {
+                fail(r->dout.status, r->dout.message);
+            }
+            out->width = r->dout.width;
+            out->height = r->dout.height;
+            out->channels = (uint32_t)r->dout.ncomp;
+        }
+        rc = 1;
+        return JP2LAM_OK;
+    });
+    return rc;
+}
+void jp2lam_server_stats(jp2lam_server* srv, uint64_t* out4) {
+    if (!srv || !out4) return;
+    ServerStats st = srv->s->stats();
+    out4[0] = st.submitted;
+    out4[1] = st.completed;
+    out4[2] = st.batches;
+    out4[3] = st.max_batch;
+}
+void jp2lam_server_free(jp2lam_server* srv) { delete srv; }
 
 }  // extern "C"

@@ -79,7 +79,7 @@ public:
     // Tier-1 encoder choice: 0 = by batch size (lane-per-block coder from
     // lane_min_blocks code-blocks up), 1 = wave per block, 2 = lane per block.
     int t1_mode = 0;
-    uint32_t lane_min_blocks = 4000;
+    uint32_t lane_min_blocks = 10000;
     bool last_t1_lanes = false;
     // Stage timings of the last call (ms, from HIP events): prepare+DWT,
     // Tier-1, PCRD+pack, host Tier-2.

@@ -1,0 +1,76 @@
+// Serving loop: continuous batching of encode and decode requests.
+//
+// `slots` workers each own an Engine (HIP stream, device and pinned
+// arenas).  A free worker takes the queue's head and then, in arrival
+// order, every further request that fits its memory budget (and the batch
+// cap), runs them as one batch, and hands results back.  Requests that
+// arrive while batches are on the GPU wait only until a slot frees; batches
+// on different slots overlap on the device, and one slot's host Tier-2
+// overlaps another's kernels.  A result is the same bytes or pixels as the
+// request alone (every stage is per image; batching only shares launches).
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace j2l {
+
+struct Request {
+    uint64_t id = 0;
+    bool is_encode = true;
+    EncItem enc;
+    std::vector<uint8_t> pixels;  // host pixels copied at submission
+    std::vector<uint8_t> stream;  // decode input copied at submission
+    size_t est_bytes = 0;         // device bytes the request is expected to hold
+    std::chrono::steady_clock::time_point t_submit, t_start, t_done;
+    int batch_size = 0;
+    bool done = false;
+    EncOut eout;
+    DecOut dout;
+};
+
+struct ServerStats {
+    uint64_t submitted = 0, completed = 0, batches = 0;
+    uint64_t max_batch = 0;
+};
+
+class Server {
+public:
+    Server(int slots, size_t mem_budget_bytes, int max_batch);
+    ~Server();
+    // Encode request: host pixels (u8, interleaved, channels 1 or 3) are
+    // copied; device pixels (on_device) must stay valid until completion.
+    uint64_t submit_encode(const uint8_t* pixels, uint32_t w, uint32_t h, uint32_t channels, int quality, int format,
+                           bool on_device);
+    uint64_t submit_decode(const uint8_t* bytes, size_t len);
+    // Waits up to timeout_ms (< 0: no limit) and moves the result out.
+    // Returns nullptr on timeout; throws Error for an unknown id.
+    std::shared_ptr<Request> wait(uint64_t id, int timeout_ms);
+    ServerStats stats();
+
+private:
+    void worker(int slot);
+    uint64_t enqueue(std::shared_ptr<Request> r);
+
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<std::shared_ptr<Request>> queue_;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> live_;
+    std::vector<std::unique_ptr<Engine>> engines_;
+    std::vector<std::thread> threads_;
+    bool stop_ = false;
+    uint64_t next_id_ = 1;
+    size_t budget_per_slot_;
+    int max_batch_;
+    ServerStats stats_;
+};
+
+}  // namespace j2l

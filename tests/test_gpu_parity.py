@@ -298,7 +298,7 @@ def test_crate_api_roundtrip(gpu):
 def lane_mode(lib):
     lib.jp2g_set_t1_mode(2, 0)
     yield lib
-    lib.jp2g_set_t1_mode(0, 4000)
+    lib.jp2g_set_t1_mode(0, 10000)
 
 
 @pytest.mark.parametrize("name,make,qs", CASES, ids=[c[0] for c in CASES])
@@ -348,4 +348,4 @@ def test_auto_mode_batch_crosses_threshold(gpu, ora, lib):
         assert j.encode_many(pages) == singles
         assert lib.jp2g_last_t1_lanes() == 0
     finally:
-        lib.jp2g_set_t1_mode(0, 4000)
+        lib.jp2g_set_t1_mode(0, 10000)
