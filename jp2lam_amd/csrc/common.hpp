@@ -73,7 +73,7 @@ struct DecodeJob {
     uint32_t data_len;
     int16_t zbp;        // -1: not included
     uint8_t passes;
-    uint8_t pad;
+    uint8_t pad;        // issue priority 0..3 (heaviest blocks highest)
 };
 
 // Per-block PCRD inputs precomputed on the host.

@@ -3,6 +3,8 @@
 // PCRD, packing) and assembles the codestreams on the host.
 #include "engine.hpp"
 
+#include <functional>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -771,7 +773,20 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     hipStream_t st = stream_;
     for (auto& im : imgs)
         J2L_HIP(hipMemcpyAsync(D + o_pay + im.pay_off, im.ci.payload, im.ci.payload_len, hipMemcpyHostToDevice, st));
-    if (nj) J2L_HIP(hipMemcpyAsync(D + o_jobs, jobs.data(), sizeof(DecodeJob) * nj, hipMemcpyHostToDevice, st));
+    if (nj) {
+        // issue priority by segment length rank: the kernel ends with its
+        // heaviest block, so those blocks win their SIMD's issue slots
+        std::vector<uint32_t> lens(nj);
+        for (int k = 0; k < nj; k++) lens[k] = jobs[k].data_len;
+        std::vector<uint32_t> sorted = lens;
+        std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
+        const uint32_t t3 = sorted[std::min<size_t>(nj - 1, (size_t)nj / 50)];
+        const uint32_t t2 = sorted[std::min<size_t>(nj - 1, (size_t)nj / 10)];
+        const uint32_t t1 = sorted[std::min<size_t>(nj - 1, (size_t)nj * 3 / 10)];
+        for (int k = 0; k < nj; k++)
+            jobs[k].pad = (uint8_t)(lens[k] >= t3 && lens[k] ? 3 : lens[k] >= t2 && lens[k] ? 2 : lens[k] >= t1 && lens[k] ? 1 : 0);
+        J2L_HIP(hipMemcpyAsync(D + o_jobs, jobs.data(), sizeof(DecodeJob) * nj, hipMemcpyHostToDevice, st));
+    }
     for (size_t q = 0; q < isegs.size(); q++)
         if (!isegs[q].empty())
             J2L_HIP(hipMemcpyAsync(D + o_iseg[q], isegs[q].data(), sizeof(DwtSeg) * isegs[q].size(), hipMemcpyHostToDevice, st));

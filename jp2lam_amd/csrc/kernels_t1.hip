@@ -31,6 +31,12 @@ __device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t val, uint32_t lane) 
     return threadIdx.x == lane ? val : v;
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// Wave issue priority within its SIMD (s_setprio takes an immediate).
+__device__ __forceinline__ void set_priority(int p) {
+    if (p >= 3) __builtin_amdgcn_s_setprio(3);
+    else if (p == 2) __builtin_amdgcn_s_setprio(2);
+    else if (p == 1) __builtin_amdgcn_s_setprio(1);
+}
 
 // Rows 4s-1..4s+4 of a column mask as bits 0..5 (missing rows read 0).
 __device__ __forceinline__ uint32_t win6(uint64_t m, int s) {
@@ -606,92 +612,123 @@ __device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t 
 // ---- MQ encoder (mq/mod.rs:163-432) --------------------------------------------
 // Coder registers in SGPRs.  cx lane k holds context k's current state as
 // its table entry (qe | nmps << 16 | nlps << 22 | switch << 28) with the
-// MPS in bit 31, so the common MPS path needs one readlane.  Output bytes
-// collect in a 64-byte VGPR window that is stored with one coalesced write.
+// MPS in bit 31, so a decision needs one readlane.  Output bytes collect in
+// a 128-byte window of two VGPRs (lane i of w0/w1 = byte wbase + i / + 64 + i);
+// the lower 64 are stored with one coalesced write once final.
 struct MqE {
     uint32_t a, c, ct, pos, last;
     uint32_t cx, tab;       // VGPRs
-    uint32_t win, wbase;    // VGPR byte window
+    uint32_t w0, w1, wbase; // byte window
     uint8_t* out;
     uint32_t cap, overflow;
 };
 
-__device__ __forceinline__ void mqe_emit(MqE& m, uint32_t b) {
-    if (m.pos > 0) {
-        const uint32_t idx = m.pos - 1u - m.wbase;
-        m.win = wlane(m.win, m.last, idx);
-        if (idx == 63u) {
-            if (m.wbase + 64u <= m.cap) m.out[m.wbase + threadIdx.x] = (uint8_t)m.win;
-            else m.overflow = 1u;
-            m.wbase += 64u;
-        }
-    }
-    m.pos++;
-    m.last = b & 0xffu;
-}
-__device__ __forceinline__ void mqe_byteout(MqE& m) {
-    if (m.last == 0xffu) {
-        mqe_emit(m, m.c >> 20);
-        m.c &= 0xfffffu;
-        m.ct = 7;
-    } else if ((m.c & 0x8000000u) == 0) {
-        mqe_emit(m, m.c >> 19);
-        m.c &= 0x7ffffu;
-        m.ct = 8;
-    } else {
-        m.last = (m.last + 1u) & 0xffu;
-        if (m.last == 0xffu) {
-            m.c &= 0x7ffffffu;
-            mqe_emit(m, m.c >> 20);
-            m.c &= 0xfffffu;
-            m.ct = 7;
-        } else {
-            mqe_emit(m, m.c >> 19);
-            m.c &= 0x7ffffu;
-            m.ct = 8;
-        }
-    }
-}
-// Renormalising path of one symbol (LPS, or MPS that leaves a < 0x8000):
-// conditional exchange, state transition, then shift until a >= 0x8000,
-// emitting a byte whenever ct runs out.
-__device__ __forceinline__ void mqe_slow(MqE& m, uint32_t e, uint32_t qe, uint32_t a, uint32_t ctx, bool is_mps) {
-    const bool to_qe = (a < qe) == is_mps;
-    m.a = to_qe ? qe : a;
-    m.c += to_qe ? 0u : qe;
-    const uint32_t next = is_mps ? (e >> 16) & 63u : (e >> 22) & 63u;
-    const uint32_t mps = is_mps ? e : e ^ (e << 3);
-    m.cx = wlane(m.cx, rl(m.tab, next) | (mps & 0x80000000u), ctx);
-    uint32_t sh = (uint32_t)__builtin_clz(m.a) - 16u;
-    if (sh < m.ct) {
-        m.a <<= sh;
-        m.c <<= sh;
-        m.ct -= sh;
-    } else {
-        do {
-            m.a <<= m.ct;
-            m.c <<= m.ct;
-            sh -= m.ct;
-            mqe_byteout(m);
-        } while (sh >= m.ct);
-        m.a <<= sh;
-        m.c <<= sh;
-        m.ct -= sh;
-    }
-}
-// One symbol; `d31` carries the decision in bit 31 (other bits ignored).
-// The common case (MPS with a still >= 0x8000) is tested with integer ops:
-// bit 31 of (a << 16) & ~(entry ^ d31) is set exactly then.
+// One decision, written as one scalar-unit sequence (mq/mod.rs:213-242 with
+// renormalisation :362-371 and byte-out :373-414): the MPS case that keeps
+// a >= 0x8000 takes 9 instructions; the MPS case that renormalises shifts by
+// exactly one bit; the LPS case shifts by the leading zeros of a.  M0 holds
+// the context index on entry (the window write reuses it).
 __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
-    const uint32_t e = rl(m.cx, ctx);
-    const uint32_t qe = e & 0xffffu;
-    const uint32_t a = m.a - qe;
-    const uint32_t x = e ^ d31;
-    if (__builtin_expect((int32_t)((a << 16) & ~x) < 0, 1)) {
-        m.a = a;
-        m.c += qe;
-    } else {
-        mqe_slow(m, e, qe, a, ctx, (int32_t)x >= 0);
+    uint32_t e, q, u, t;
+    asm volatile(
+        "v_readlane_b32 %[e], %[cx], m0\n\t"
+        "s_and_b32 %[q], %[e], 0xffff\n\t"
+        "s_sub_u32 %[a], %[a], %[q]\n\t"
+        "s_xor_b32 %[u], %[e], %[d]\n\t"
+        "s_lshl_b32 %[t], %[a], 16\n\t"
+        "s_andn2_b32 %[t], %[t], %[u]\n\t"
+        "s_cmp_lt_i32 %[t], 0\n\t"
+        "s_cbranch_scc0 .Lslow%=\n\t"
+        "s_add_u32 %[c], %[c], %[q]\n\t"
+        "s_branch .Ldone%=\n"
+        ".Lslow%=:\n\t"
+        "s_cmp_lt_i32 %[u], 0\n\t"
+        "s_cbranch_scc1 .Llps%=\n\t"
+        // MPS, a < 0x8000: a = qe if a < qe, else c += qe; state NMPS
+        "s_cmp_lt_u32 %[a], %[q]\n\t"
+        "s_cselect_b32 %[t], 0, %[q]\n\t"
+        "s_cselect_b32 %[a], %[q], %[a]\n\t"
+        "s_add_u32 %[c], %[c], %[t]\n\t"
+        "s_bfe_u32 %[q], %[e], 0x60010\n\t"
+        "v_readlane_b32 %[t], %[tab], %[q]\n\t"
+        "s_and_b32 %[e], %[e], 0x80000000\n\t"
+        "s_or_b32 %[t], %[t], %[e]\n\t"
+        "v_writelane_b32 %[cx], %[t], m0\n\t"
+        "s_mov_b32 %[t], 1\n\t"
+        "s_branch .Lrn%=\n"
+        ".Llps%=:\n\t"
+        // LPS: c += qe if a < qe, else a = qe; state NLPS, MPS ^= SWITCH
+        "s_cmp_lt_u32 %[a], %[q]\n\t"
+        "s_cselect_b32 %[t], %[q], 0\n\t"
+        "s_cselect_b32 %[a], %[a], %[q]\n\t"
+        "s_add_u32 %[c], %[c], %[t]\n\t"
+        "s_bfe_u32 %[q], %[e], 0x60016\n\t"
+        "v_readlane_b32 %[t], %[tab], %[q]\n\t"
+        "s_lshl_b32 %[u], %[e], 3\n\t"
+        "s_xor_b32 %[u], %[u], %[e]\n\t"
+        "s_and_b32 %[u], %[u], 0x80000000\n\t"
+        "s_or_b32 %[t], %[t], %[u]\n\t"
+        "v_writelane_b32 %[cx], %[t], m0\n\t"
+        "s_flbit_i32_b32 %[t], %[a]\n\t"
+        "s_sub_u32 %[t], %[t], 16\n"
+        // renormalise by t bits, a byte out whenever ct reaches zero
+        ".Lrn%=:\n\t"
+        "s_cmp_lt_u32 %[t], %[ct]\n\t"
+        "s_cbranch_scc1 .Lrnlast%=\n\t"
+        "s_lshl_b32 %[a], %[a], %[ct]\n\t"
+        "s_lshl_b32 %[c], %[c], %[ct]\n\t"
+        "s_sub_u32 %[t], %[t], %[ct]\n\t"
+        "s_cmp_eq_u32 %[last], 0xff\n\t"
+        "s_cbranch_scc1 .Lbff%=\n\t"
+        "s_bitcmp1_b32 %[c], 27\n\t"
+        "s_cbranch_scc0 .Lbplain%=\n\t"
+        "s_add_u32 %[last], %[last], 1\n\t"
+        "s_cmp_eq_u32 %[last], 0xff\n\t"
+        "s_cbranch_scc0 .Lbplain%=\n\t"
+        "s_bitset0_b32 %[c], 27\n"
+        ".Lbff%=:\n\t"
+        "s_lshr_b32 %[q], %[c], 20\n\t"
+        "s_and_b32 %[c], %[c], 0xfffff\n\t"
+        "s_mov_b32 %[ct], 7\n\t"
+        "s_branch .Lemit%=\n"
+        ".Lbplain%=:\n\t"
+        "s_lshr_b32 %[q], %[c], 19\n\t"
+        "s_and_b32 %[c], %[c], 0x7ffff\n\t"
+        "s_mov_b32 %[ct], 8\n"
+        // commit the held byte at window index pos - 1 - wbase, hold the new one
+        ".Lemit%=:\n\t"
+        "s_cmp_eq_u32 %[pos], 0\n\t"
+        "s_cbranch_scc1 .Lefirst%=\n\t"
+        "s_sub_u32 %[u], %[pos], %[wb]\n\t"
+        "s_sub_u32 m0, %[u], 1\n\t"
+        "s_cmp_lt_u32 m0, 64\n\t"
+        "s_cbranch_scc0 .Lehi%=\n\t"
+        "v_writelane_b32 %[w0], %[last], m0\n\t"
+        "s_branch .Lefirst%=\n"
+        ".Lehi%=:\n\t"
+        "v_writelane_b32 %[w1], %[last], m0\n"
+        ".Lefirst%=:\n\t"
+        "s_add_u32 %[pos], %[pos], 1\n\t"
+        "s_and_b32 %[last], %[q], 0xff\n\t"
+        "s_branch .Lrn%=\n"
+        ".Lrnlast%=:\n\t"
+        "s_lshl_b32 %[a], %[a], %[t]\n\t"
+        "s_lshl_b32 %[c], %[c], %[t]\n\t"
+        "s_sub_u32 %[ct], %[ct], %[t]\n"
+        ".Ldone%=:"
+        : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
+          [cx] "+v"(m.cx), [w0] "+v"(m.w0), [w1] "+v"(m.w1), [m0] "+{m0}"(ctx), [e] "=&s"(e), [q] "=&s"(q),
+          [u] "=&s"(u), [t] "=&s"(t)
+        : [d] "s"(d31), [tab] "v"(m.tab), [wb] "s"(m.wbase)
+        : "scc");
+}
+// Store the window's lower 64 bytes once they are all committed.
+__device__ __forceinline__ void mqe_spill(MqE& m) {
+    if (m.pos > m.wbase + 64u) {
+        if (m.wbase + 64u <= m.cap) m.out[m.wbase + threadIdx.x] = (uint8_t)m.w0;
+        else m.overflow = 1u;
+        m.w0 = m.w1;
+        m.wbase += 64u;
     }
 }
 // Code the n symbol bytes of one pass from LDS.
@@ -706,16 +743,51 @@ __device__ __forceinline__ void mqe_pass(MqE& m, const uint32_t* symw, uint32_t 
             mqe_symbol(m, (w >> 8) & 31u, w << 16);
             mqe_symbol(m, (w >> 16) & 31u, w << 8);
             mqe_symbol(m, (w >> 24) & 31u, w);
+            mqe_spill(m);
         }
         if (q < cnt) {
             uint32_t w = rl(sv, q >> 2);
             for (; q < cnt; q++, w >>= 8) mqe_symbol(m, w & 31u, w << 24);
+            mqe_spill(m);
         }
     }
+}
+// Byte-out for the flush (same as the asm sequence above).
+__device__ __forceinline__ void mqe_byteout(MqE& m) {
+    uint32_t b;
+    if (m.last == 0xffu) {
+        b = m.c >> 20;
+        m.c &= 0xfffffu;
+        m.ct = 7;
+    } else if ((m.c & 0x8000000u) == 0) {
+        b = m.c >> 19;
+        m.c &= 0x7ffffu;
+        m.ct = 8;
+    } else {
+        m.last = (m.last + 1u) & 0xffu;
+        if (m.last == 0xffu) {
+            m.c &= 0x7ffffffu;
+            b = m.c >> 20;
+            m.c &= 0xfffffu;
+            m.ct = 7;
+        } else {
+            b = m.c >> 19;
+            m.c &= 0x7ffffu;
+            m.ct = 8;
+        }
+    }
+    if (m.pos > 0) {
+        const uint32_t idx = m.pos - 1u - m.wbase;
+        if (idx < 64u) m.w0 = wl(m.w0, m.last, idx);
+        else m.w1 = wl(m.w1, m.last, idx - 64u);
+    }
+    m.pos++;
+    m.last = b & 0xffu;
 }
 // flush + take_bytes (mq/mod.rs:244-250, :373-379, :416-423); returns the
 // terminated length.
 __device__ __forceinline__ uint32_t mqe_flush(MqE& m) {
+    mqe_spill(m);
     uint32_t temp = m.c + m.a;
     m.c |= 0xffffu;
     if (m.c >= temp) m.c -= 0x8000u;
@@ -724,11 +796,22 @@ __device__ __forceinline__ uint32_t mqe_flush(MqE& m) {
     m.c <<= m.ct;
     mqe_byteout(m);
     const uint32_t total = m.last == 0xffu ? m.pos - 1u : m.pos;
-    if (m.pos > 0) m.win = wlane(m.win, m.last, m.pos - 1u - m.wbase);
-    const uint32_t nwin = m.pos - m.wbase;
-    if (threadIdx.x < nwin) {
-        const uint32_t o = m.wbase + threadIdx.x;
-        if (o < m.cap) m.out[o] = (uint8_t)m.win;
+    {
+        const uint32_t idx = m.pos - 1u - m.wbase;  // commit the held byte
+        if (idx < 64u) m.w0 = wl(m.w0, m.last, idx);
+        else m.w1 = wl(m.w1, m.last, idx - 64u);
+    }
+    const uint32_t nwin = m.pos - m.wbase;  // <= 128 bytes pending
+    const uint32_t lane = threadIdx.x;
+    if (lane < nwin) {
+        const uint32_t o = m.wbase + lane;
+        if (o < m.cap) m.out[o] = (uint8_t)m.w0;
+        else m.overflow = 1u;
+    }
+    if (lane + 64u < nwin) {
+        const uint32_t o = m.wbase + 64u + lane;
+        if (o < m.cap) m.out[o] = (uint8_t)m.w1;
+        else m.overflow = 1u;
     }
     if (m.wbase + nwin > m.cap) m.overflow = 1u;
     return total;
@@ -805,6 +888,9 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
         if (lane == 0) results[blockIdx.x] = R;
         return;
     }
+    // The kernel ends with its heaviest block; blocks with many bit-planes
+    // get issue priority over the light block sharing their SIMD.
+    set_priority(mbp >= 12 ? 3 : mbp >= 10 ? 2 : mbp >= 8 ? 1 : 0);
     Tables t;
     load_tables(t, tabs, J.band);
     MqE m;
@@ -816,7 +902,8 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     m.tab = t.mq;
     // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
     m.cx = (uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64);
-    m.win = 0;
+    m.w0 = 0;
+    m.w1 = 0;
     m.wbase = 0;
     m.out = out + J.out_off;
     m.cap = J.out_cap;
@@ -1146,6 +1233,7 @@ __global__ __launch_bounds__(64) void t1_decode_kernel(const DecodeJob* __restri
 #pragma unroll
     for (int y = 0; y < 64; y++) mag[y] = 0;
     uint64_t sgnmask = 0;
+    set_priority(J.pad);  // the host ranks blocks by segment length
     if (J.zbp >= 0 && J.passes > 0) {
         Tables t;
         load_tables(t, tabs, J.band);
