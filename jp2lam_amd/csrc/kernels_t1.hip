@@ -52,83 +52,157 @@ struct Tables {
 __device__ __forceinline__ uint32_t zc_ctx(const Tables& t, uint32_t nb) { return (rl(t.zc, nb >> 3) >> ((nb & 7) * 4)) & 15u; }
 __device__ __forceinline__ uint32_t sc_entry(const Tables& t, uint32_t si) { return (rl(t.sc, si >> 2) >> ((si & 3) * 8)) & 255u; }
 
-// 94-state index s = 2 * iso_state + mps; reset values of mq/mod.rs:195-200.
-__device__ __forceinline__ uint32_t init_ctx_lane(int lane) {
-    return lane == CTX_UNI ? 2u * 46 : lane == CTX_AGG ? 2u * 3 : lane == CTX_ZC ? 2u * 4 : 0u;
-}
 
 // ---- MQ decoder (mq/mod.rs:30-155) -----------------------------------------
+// Coder registers in SGPRs; context lanes hold the state's table entry with
+// the MPS in bit 31 (as in the encoder).  The segment is read through a
+// 128-byte window of two VGPRs (lane i of w0/w1 = byte dbase + i / + 64 + i,
+// 0xFF past the end, which the decoder reads as the reference does,
+// mq/mod.rs:152-154); the scan slides it before each column, which reads
+// fewer than 24 bytes, so a decision never leaves it.
 struct MqDec {
-    uint32_t a, c, ct, bp;
-    uint32_t ctxv;
-    uint32_t win, wbase;  // VGPR byte window over the segment
+    uint32_t a, c, ct, bp, cur;  // cur = byte at bp
+    uint32_t cx, tab;            // VGPRs
+    uint32_t w0, w1, dbase;
     const uint8_t* data;
     uint32_t len;
 };
-__device__ __forceinline__ void dec_load(MqDec& d, uint32_t base) {
-    d.wbase = base;
-    uint32_t i = base + threadIdx.x;
-    d.win = i < d.len ? (uint32_t)d.data[i] : 0xffu;
+__device__ __forceinline__ uint32_t dec_load64(const MqDec& d, uint32_t base) {
+    const uint32_t i = base + threadIdx.x;
+    return i < d.len ? (uint32_t)d.data[i] : 0xffu;
 }
-__device__ __forceinline__ uint32_t dec_byte(MqDec& d, uint32_t i) {
-    if (i >= d.len) return 0xffu;
-    if (i < d.wbase || i >= d.wbase + 64) dec_load(d, i);
-    return rl(d.win, i - d.wbase);
+__device__ __forceinline__ void dec_window_init(MqDec& d) {
+    d.dbase = 0;
+    d.w0 = dec_load64(d, 0);
+    d.w1 = dec_load64(d, 64);
 }
-__device__ __forceinline__ void dec_bytein(MqDec& d) {
-    uint32_t next = dec_byte(d, d.bp + 1);
-    if (dec_byte(d, d.bp) == 0xffu) {
+__device__ __forceinline__ void dec_slide(MqDec& d) {
+    if (d.bp + 24u >= d.dbase + 128u) {
+        d.w0 = d.w1;
+        d.dbase += 64u;
+        d.w1 = dec_load64(d, d.dbase + 64u);
+    }
+}
+// Start of decoding (mq/mod.rs:46-60): c from the first byte, one byte-in,
+// then c <<= 7.
+__device__ __forceinline__ void dec_start(MqDec& d) {
+    dec_window_init(d);
+    d.bp = 0;
+    d.a = 0x8000u;
+    d.cur = rl(d.w0, 0);
+    d.c = d.cur << 16;
+    const uint32_t next = rl(d.w0, 1);
+    if (d.cur == 0xffu) {
         if (next > 0x8fu) {
             d.c += 0xff00u;
             d.ct = 8;
         } else {
-            d.bp++;
+            d.bp = 1;
+            d.cur = next;
             d.c += next << 9;
             d.ct = 7;
         }
     } else {
-        d.bp++;
+        d.bp = 1;
+        d.cur = next;
         d.c += next << 8;
         d.ct = 8;
     }
+    d.c <<= 7;
+    d.ct = d.ct >= 7 ? d.ct - 7 : 0;
 }
-__device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables& t, uint32_t ctx) {
-    uint32_t s = rl(d.ctxv, ctx);
-    uint32_t e = rl(t.mq, s >> 1);
-    uint32_t qe = e & 0xffffu, mps = s & 1u, bit;
-    uint32_t nmps = (((e >> 16) & 63u) << 1) | mps;
-    uint32_t nlps = (((e >> 22) & 63u) << 1) | (mps ^ ((e >> 28) & 1u));
-    d.a -= qe;
-    if ((d.c >> 16) < qe) {
-        uint32_t post = d.a;
-        d.a = qe;
-        if (post < qe) {
-            d.ctxv = wl(d.ctxv, nmps, ctx);
-            bit = mps;
-        } else {
-            d.ctxv = wl(d.ctxv, nlps, ctx);
-            bit = mps ^ 1u;
-        }
-    } else {
-        d.c -= qe << 16;
-        if ((d.a & 0x8000u) == 0) {
-            if (d.a < qe) {
-                d.ctxv = wl(d.ctxv, nlps, ctx);
-                bit = mps ^ 1u;
-            } else {
-                d.ctxv = wl(d.ctxv, nmps, ctx);
-                bit = mps;
-            }
-        } else {
-            bit = mps;
-        }
-    }
-    while (d.a < 0x8000u) {
-        if (d.ct == 0) dec_bytein(d);
-        d.a <<= 1;
-        d.c <<= 1;
-        d.ct--;
-    }
+// One decision as one scalar-unit sequence (mq/mod.rs:62-155): M0 holds the
+// context index on entry.
+__device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t ctx) {
+    uint32_t bit, e, q, t, k, n0, n1;
+    asm volatile(
+        "v_readlane_b32 %[e], %[cx], m0\n\t"
+        "s_and_b32 %[q], %[e], 0xffff\n\t"
+        "s_sub_u32 %[a], %[a], %[q]\n\t"
+        "s_lshr_b32 %[t], %[c], 16\n\t"
+        "s_cmp_lt_u32 %[t], %[q]\n\t"
+        "s_cbranch_scc1 .Llower%=\n\t"
+        "s_lshl_b32 %[t], %[q], 16\n\t"
+        "s_sub_u32 %[c], %[c], %[t]\n\t"
+        "s_bitcmp1_b32 %[a], 15\n\t"
+        "s_cbranch_scc0 .Lup%=\n\t"
+        "s_lshr_b32 %[bit], %[e], 31\n\t"
+        "s_branch .Ldone%=\n"
+        ".Lup%=:\n\t"                       // upper sub-interval, a < 0x8000: LPS iff a < qe
+        "s_cmp_lt_u32 %[a], %[q]\n\t"
+        "s_cbranch_scc1 .Llps%=\n\t"
+        "s_branch .Lmps%=\n"
+        ".Llower%=:\n\t"                    // lower sub-interval: a = qe; MPS iff a < qe
+        "s_cmp_lt_u32 %[a], %[q]\n\t"
+        "s_mov_b32 %[a], %[q]\n\t"
+        "s_cbranch_scc0 .Llps%=\n"
+        ".Lmps%=:\n\t"
+        "s_lshr_b32 %[bit], %[e], 31\n\t"
+        "s_bfe_u32 %[t], %[e], 0x60010\n\t"
+        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "s_and_b32 %[k], %[e], 0x80000000\n\t"
+        "s_or_b32 %[t], %[t], %[k]\n\t"
+        "v_writelane_b32 %[cx], %[t], m0\n\t"
+        "s_branch .Lrenorm%=\n"
+        ".Llps%=:\n\t"
+        "s_lshr_b32 %[bit], %[e], 31\n\t"
+        "s_xor_b32 %[bit], %[bit], 1\n\t"
+        "s_bfe_u32 %[t], %[e], 0x60016\n\t"
+        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "s_lshl_b32 %[k], %[e], 3\n\t"
+        "s_xor_b32 %[k], %[k], %[e]\n\t"
+        "s_and_b32 %[k], %[k], 0x80000000\n\t"
+        "s_or_b32 %[t], %[t], %[k]\n\t"
+        "v_writelane_b32 %[cx], %[t], m0\n"
+        ".Lrenorm%=:\n\t"                   // shift until a >= 0x8000, a byte in whenever ct is 0
+        "s_flbit_i32_b32 %[t], %[a]\n\t"
+        "s_sub_u32 %[t], %[t], 16\n"
+        ".Lrn%=:\n\t"
+        "s_cmp_eq_u32 %[ct], 0\n\t"
+        "s_cbranch_scc1 .Lbytein%=\n"
+        ".Lrn2%=:\n\t"
+        "s_min_u32 %[k], %[t], %[ct]\n\t"
+        "s_lshl_b32 %[a], %[a], %[k]\n\t"
+        "s_lshl_b32 %[c], %[c], %[k]\n\t"
+        "s_sub_u32 %[ct], %[ct], %[k]\n\t"
+        "s_sub_u32 %[t], %[t], %[k]\n\t"
+        "s_cmp_eq_u32 %[t], 0\n\t"
+        "s_cbranch_scc0 .Lrn%=\n\t"
+        "s_branch .Ldone%=\n"
+        ".Lbytein%=:\n\t"                   // mq/mod.rs:133-155
+        "s_sub_u32 %[k], %[bp], %[db]\n\t"
+        "s_add_u32 %[k], %[k], 1\n\t"
+        "v_readlane_b32 %[n0], %[w0], %[k]\n\t"
+        "v_readlane_b32 %[n1], %[w1], %[k]\n\t"
+        "s_cmp_lt_u32 %[k], 64\n\t"
+        "s_cselect_b32 %[n0], %[n0], %[n1]\n\t"
+        "s_cmp_eq_u32 %[cur], 0xff\n\t"
+        "s_cbranch_scc0 .Lbplain%=\n\t"
+        "s_cmp_gt_u32 %[n0], 0x8f\n\t"
+        "s_cbranch_scc0 .Lbstuff%=\n\t"
+        "s_add_u32 %[c], %[c], 0xff00\n\t"
+        "s_mov_b32 %[ct], 8\n\t"
+        "s_branch .Lrn2%=\n"
+        ".Lbstuff%=:\n\t"
+        "s_add_u32 %[bp], %[bp], 1\n\t"
+        "s_mov_b32 %[cur], %[n0]\n\t"
+        "s_lshl_b32 %[n0], %[n0], 9\n\t"
+        "s_add_u32 %[c], %[c], %[n0]\n\t"
+        "s_mov_b32 %[ct], 7\n\t"
+        "s_branch .Lrn2%=\n"
+        ".Lbplain%=:\n\t"
+        "s_add_u32 %[bp], %[bp], 1\n\t"
+        "s_mov_b32 %[cur], %[n0]\n\t"
+        "s_lshl_b32 %[n0], %[n0], 8\n\t"
+        "s_add_u32 %[c], %[c], %[n0]\n\t"
+        "s_mov_b32 %[ct], 8\n\t"
+        "s_branch .Lrn2%=\n"
+        ".Ldone%=:"
+        : [a] "+s"(d.a), [c] "+s"(d.c), [ct] "+s"(d.ct), [bp] "+s"(d.bp), [cur] "+s"(d.cur), [cx] "+v"(d.cx),
+          [m0] "+{m0}"(ctx), [bit] "=&s"(bit), [e] "=&s"(e), [q] "=&s"(q), [t] "=&s"(t), [k] "=&s"(k),
+          [n0] "=&s"(n0), [n1] "=&s"(n1)
+        : [tab] "v"(d.tab), [w0] "v"(d.w0), [w1] "v"(d.w1), [db] "s"(d.dbase)
+        : "scc");
     return bit;
 }
 
@@ -163,6 +237,7 @@ template <>
 struct Coder<true> {
     MqDec m;
     __device__ __forceinline__ uint32_t code(const Tables& t, uint32_t ctx, uint32_t) { return mq_decode(m, t, ctx); }
+    __device__ __forceinline__ void column() { dec_slide(m); }
 };
 
 // Significance propagation pass over the block (t1.rs:805-905,
@@ -193,6 +268,7 @@ __device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
         uint32_t newW = 0, newWs = 0;
         int x = M ? __builtin_ctzll(M) : 64;
         while (x < w) {
+            cd.column();
             uint32_t Pc = rl(P, x), Pw = x > 0 ? rl(P, x - 1) : 0u, Pe = x < 63 ? rl(P, x + 1) : 0u;
             uint32_t sigC = Pc & 63u, sgnC = (Pc >> 6) & 63u, bC = (Pc >> 12) & 15u, nC = (Pc >> 16) & 15u;
             uint32_t sigW = (Pw & 63u) | (newW << 1), sgnW = ((Pw >> 6) & 63u) | (newWs << 1);
@@ -260,6 +336,7 @@ __device__ void mr_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits,
         while (M) {
             int x = __builtin_ctzll(M);
             M &= M - 1;
+            cd.column();
             uint32_t q = rl(Q, x), ones = 0;
             for (int ci = 0; ci < nrows; ci++) {
                 if (!((q >> ci) & 1u)) continue;
@@ -306,6 +383,7 @@ __device__ uint32_t cl_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
                 newW = 0;
                 newWs = 0;
             }
+            cd.column();
             uint32_t Pc = rl(P, x), Pw = x > 0 ? rl(P, x - 1) : 0u, Pe = x < 63 ? rl(P, x + 1) : 0u;
             uint32_t sigC = Pc & 63u, sgnC = (Pc >> 6) & 63u, bC = (Pc >> 12) & 15u, nC = (Pc >> 16) & 15u;
             uint32_t visC = (Pc >> 20) & 15u;
@@ -1240,15 +1318,10 @@ __global__ __launch_bounds__(64) void t1_decode_kernel(const DecodeJob* __restri
         Coder<true> cd;
         cd.m.data = payload + J.data_off;
         cd.m.len = J.data_len;
-        cd.m.ctxv = init_ctx_lane(lane);
-        dec_load(cd.m, 0);
-        cd.m.bp = 0;
-        cd.m.a = 0x8000;
-        cd.m.c = (J.data_len == 0 ? 0xffu : dec_byte(cd.m, 0)) << 16;
-        cd.m.ct = 0;
-        dec_bytein(cd.m);
-        cd.m.c <<= 7;
-        cd.m.ct = cd.m.ct >= 7 ? cd.m.ct - 7 : 0;
+        cd.m.tab = t.mq;
+        // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
+        cd.m.cx = (uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64);
+        dec_start(cd.m);
         Col st = {0, 0, 0, 0};
         const int mbp = (int)J.mb - (int)J.zbp;
         int remaining = J.passes;
