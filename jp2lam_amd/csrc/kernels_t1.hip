@@ -111,8 +111,10 @@ __device__ __forceinline__ void dec_start(MqDec& d) {
     d.c <<= 7;
     d.ct = d.ct >= 7 ? d.ct - 7 : 0;
 }
-// One decision as one scalar-unit sequence (mq/mod.rs:62-155): M0 holds the
-// context index on entry.
+// One decision as one scalar-unit sequence (mq/mod.rs:62-155): the MPS
+// case that keeps a >= 0x8000 exits early; otherwise the LPS flag becomes a
+// mask and one common tail updates the state, forms the decision and shifts
+// (byte-in only when ct runs out).  M0 holds the context index on entry.
 __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t ctx) {
     uint32_t bit, e, q, t, k, n0, n1;
     asm volatile(
@@ -128,36 +130,38 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_cbranch_scc0 .Lup%=\n\t"
         "s_lshr_b32 %[bit], %[e], 31\n\t"
         "s_branch .Ldone%=\n"
-        ".Lup%=:\n\t"                       // upper sub-interval, a < 0x8000: LPS iff a < qe
+        ".Lup%=:\n\t"                        // upper sub-interval, a < 0x8000: LPS iff a < qe
         "s_cmp_lt_u32 %[a], %[q]\n\t"
-        "s_cbranch_scc1 .Llps%=\n\t"
-        "s_branch .Lmps%=\n"
-        ".Llower%=:\n\t"                    // lower sub-interval: a = qe; MPS iff a < qe
+        "s_cselect_b32 %[k], -1, 0\n\t"
+        "s_branch .Lcommon%=\n"
+        ".Llower%=:\n\t"                     // lower sub-interval: a = qe; LPS iff a >= qe
         "s_cmp_lt_u32 %[a], %[q]\n\t"
-        "s_mov_b32 %[a], %[q]\n\t"
-        "s_cbranch_scc0 .Llps%=\n"
-        ".Lmps%=:\n\t"
+        "s_cselect_b32 %[k], 0, -1\n\t"
+        "s_mov_b32 %[a], %[q]\n"
+        ".Lcommon%=:\n\t"                    // k = all ones on LPS
         "s_lshr_b32 %[bit], %[e], 31\n\t"
-        "s_bfe_u32 %[t], %[e], 0x60010\n\t"
+        "s_and_b32 %[t], %[k], 1\n\t"
+        "s_xor_b32 %[bit], %[bit], %[t]\n\t"
+        "s_lshl_b32 %[n0], %[e], 3\n\t"
+        "s_and_b32 %[n0], %[n0], %[k]\n\t"
+        "s_xor_b32 %[n0], %[n0], %[e]\n\t"
+        "s_and_b32 %[n0], %[n0], 0x80000000\n\t"
+        "s_cmp_lg_u32 %[k], 0\n\t"
+        "s_cselect_b32 %[t], 22, 16\n\t"
+        "s_lshr_b32 %[t], %[e], %[t]\n\t"
+        "s_and_b32 %[t], %[t], 63\n\t"
         "v_readlane_b32 %[t], %[tab], %[t]\n\t"
-        "s_and_b32 %[k], %[e], 0x80000000\n\t"
-        "s_or_b32 %[t], %[t], %[k]\n\t"
+        "s_or_b32 %[t], %[t], %[n0]\n\t"
         "v_writelane_b32 %[cx], %[t], m0\n\t"
-        "s_branch .Lrenorm%=\n"
-        ".Llps%=:\n\t"
-        "s_lshr_b32 %[bit], %[e], 31\n\t"
-        "s_xor_b32 %[bit], %[bit], 1\n\t"
-        "s_bfe_u32 %[t], %[e], 0x60016\n\t"
-        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
-        "s_lshl_b32 %[k], %[e], 3\n\t"
-        "s_xor_b32 %[k], %[k], %[e]\n\t"
-        "s_and_b32 %[k], %[k], 0x80000000\n\t"
-        "s_or_b32 %[t], %[t], %[k]\n\t"
-        "v_writelane_b32 %[cx], %[t], m0\n"
-        ".Lrenorm%=:\n\t"                   // shift until a >= 0x8000, a byte in whenever ct is 0
         "s_flbit_i32_b32 %[t], %[a]\n\t"
-        "s_sub_u32 %[t], %[t], 16\n"
-        ".Lrn%=:\n\t"
+        "s_sub_u32 %[t], %[t], 16\n\t"
+        "s_cmp_gt_u32 %[t], %[ct]\n\t"
+        "s_cbranch_scc1 .Lrn%=\n\t"
+        "s_lshl_b32 %[a], %[a], %[t]\n\t"
+        "s_lshl_b32 %[c], %[c], %[t]\n\t"
+        "s_sub_u32 %[ct], %[ct], %[t]\n\t"
+        "s_branch .Ldone%=\n"
+        ".Lrn%=:\n\t"                        // shift with byte-in whenever ct is 0
         "s_cmp_eq_u32 %[ct], 0\n\t"
         "s_cbranch_scc1 .Lbytein%=\n"
         ".Lrn2%=:\n\t"
@@ -169,7 +173,7 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_cmp_eq_u32 %[t], 0\n\t"
         "s_cbranch_scc0 .Lrn%=\n\t"
         "s_branch .Ldone%=\n"
-        ".Lbytein%=:\n\t"                   // mq/mod.rs:133-155
+        ".Lbytein%=:\n\t"                    // mq/mod.rs:133-155
         "s_sub_u32 %[k], %[bp], %[db]\n\t"
         "s_add_u32 %[k], %[k], 1\n\t"
         "v_readlane_b32 %[n0], %[w0], %[k]\n\t"
@@ -720,32 +724,27 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_add_u32 %[c], %[c], %[q]\n\t"
         "s_branch .Ldone%=\n"
         ".Lslow%=:\n\t"
-        "s_cmp_lt_i32 %[u], 0\n\t"
-        "s_cbranch_scc1 .Llps%=\n\t"
-        // MPS, a < 0x8000: a = qe if a < qe, else c += qe; state NMPS
+        // renormalising decision, MPS or LPS, branch-free: conditional
+        // exchange to_qe = (a < qe) XOR lps; MPS: a = qe if a < qe else
+        // c += qe; LPS: c += qe if a < qe else a = qe; state NMPS / NLPS,
+        // MPS ^= SWITCH on LPS
         "s_cmp_lt_u32 %[a], %[q]\n\t"
-        "s_cselect_b32 %[t], 0, %[q]\n\t"
+        "s_cselect_b32 %[t], -1, 0\n\t"
+        "s_xor_b32 %[t], %[t], %[u]\n\t"
+        "s_cmp_lt_i32 %[t], 0\n\t"
         "s_cselect_b32 %[a], %[q], %[a]\n\t"
+        "s_cselect_b32 %[t], 0, %[q]\n\t"
         "s_add_u32 %[c], %[c], %[t]\n\t"
-        "s_bfe_u32 %[q], %[e], 0x60010\n\t"
-        "v_readlane_b32 %[t], %[tab], %[q]\n\t"
-        "s_and_b32 %[e], %[e], 0x80000000\n\t"
-        "s_or_b32 %[t], %[t], %[e]\n\t"
-        "v_writelane_b32 %[cx], %[t], m0\n\t"
-        "s_mov_b32 %[t], 1\n\t"
-        "s_branch .Lrn%=\n"
-        ".Llps%=:\n\t"
-        // LPS: c += qe if a < qe, else a = qe; state NLPS, MPS ^= SWITCH
-        "s_cmp_lt_u32 %[a], %[q]\n\t"
-        "s_cselect_b32 %[t], %[q], 0\n\t"
-        "s_cselect_b32 %[a], %[a], %[q]\n\t"
-        "s_add_u32 %[c], %[c], %[t]\n\t"
-        "s_bfe_u32 %[q], %[e], 0x60016\n\t"
-        "v_readlane_b32 %[t], %[tab], %[q]\n\t"
-        "s_lshl_b32 %[u], %[e], 3\n\t"
-        "s_xor_b32 %[u], %[u], %[e]\n\t"
-        "s_and_b32 %[u], %[u], 0x80000000\n\t"
-        "s_or_b32 %[t], %[t], %[u]\n\t"
+        "s_lshl_b32 %[q], %[e], 3\n\t"
+        "s_cmp_lt_i32 %[u], 0\n\t"
+        "s_cselect_b32 %[t], 22, 16\n\t"
+        "s_cselect_b32 %[q], %[q], 0\n\t"
+        "s_xor_b32 %[q], %[q], %[e]\n\t"
+        "s_lshr_b32 %[t], %[e], %[t]\n\t"
+        "s_and_b32 %[t], %[t], 63\n\t"
+        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "s_and_b32 %[q], %[q], 0x80000000\n\t"
+        "s_or_b32 %[t], %[t], %[q]\n\t"
         "v_writelane_b32 %[cx], %[t], m0\n\t"
         "s_flbit_i32_b32 %[t], %[a]\n\t"
         "s_sub_u32 %[t], %[t], 16\n"
