@@ -205,10 +205,13 @@ void jp2g_last_block_passes(int index, int32_t* out);
 void jp2g_last_block_bytes(int index, uint8_t* out);
 double jp2g_last_lambda(int group);
 /* Tier-1 encoder selection (no reference counterpart; the reference codes
- * every block the same way, t1.rs:287-290): mode 0 = by batch size, the
- * lane-per-block coder from `lane_min_blocks` code-blocks up; 1 = one
- * wavefront per code-block; 2 = one lane per code-block.  Output is the
- * same in every mode. */
+ * every block the same way, t1.rs:287-290): mode 0 = tuned: the first batch
+ * of each size class (bit length of its code-block count) from
+ * `lane_min_blocks` up runs both coders and the faster is kept for the
+ * class, smaller batches use one wavefront per code-block; 1 = one
+ * wavefront per code-block; 2 = one lane per code-block; 3 = one lane per
+ * code-block from `lane_min_blocks` code-blocks up.  Setting the mode
+ * clears the tuned choices.  Output is the same in every mode. */
 void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks);
 /* 1 when the last encode ran the lane-per-block coder. */
 int jp2g_last_t1_lanes(void);

@@ -666,7 +666,8 @@ void jp2g_set_debug(int on) { engine().keep_debug = on != 0; }
 void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks) {
     Engine& e = engine();
     std::lock_guard<std::mutex> g(e.mu);
-    e.t1_mode = mode < 0 || mode > 2 ? 0 : mode;
+    e.t1_mode = mode < 0 || mode > 3 ? 0 : mode;
+    e.tuned_.clear();
     if (lane_min_blocks) e.lane_min_blocks = lane_min_blocks;
 }
 int jp2g_last_t1_lanes(void) { return engine().last_t1_lanes ? 1 : 0; }

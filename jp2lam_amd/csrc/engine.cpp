@@ -267,7 +267,22 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     // Lane-per-block Tier-1 (large batches): symbol streams, bounded per
     // block by the band's Mb + 2 bit-planes; an overflow falls back to the
     // wave-per-block kernel below.
-    const bool lanes = t1_mode == 2 || (t1_mode == 0 && (uint32_t)nb >= lane_min_blocks);
+    // Tier-1 coder for this batch: forced (1 wave / 2 lane per block), a
+    // fixed threshold (3), or tuned (0): the first batch of each size class
+    // (bit length of the block count) from lane_min_blocks up runs both
+    // coders on the same coefficients and keeps the faster; smaller batches
+    // use the wave coder.
+    const int bucket = nb ? 32 - __builtin_clz((unsigned)nb) : 0;
+    int coder = t1_mode == 1 || t1_mode == 2 ? t1_mode : 1;
+    bool probe = false;
+    if (t1_mode == 3) {
+        coder = (uint32_t)nb >= lane_min_blocks ? 2 : 1;
+    } else if (t1_mode == 0 && (uint32_t)nb >= lane_min_blocks) {
+        auto it = tuned_.find(bucket);
+        if (it != tuned_.end()) coder = it->second;
+        else probe = true;
+    }
+    const bool lanes = coder == 2 || probe;
     size_t o_pend = 0, o_soff = 0, o_pused = 0, o_pool = 0;
     unsigned long long pool_cap = 0;
     if (lanes) {
@@ -395,6 +410,14 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                 J2L_HIP(hipMemcpyAsync(D + im.p_off[c], D + im.x_off[c], im.n * 4, hipMemcpyDeviceToDevice, st));
     const unsigned long long zero64 = 0;
     bool use_lanes = lanes;
+    if (probe) {
+        // wave coder timed first; the lane coder then runs below and its
+        // (identical) results are the ones used
+        J2L_HIP(hipEventRecord(ev.e[4], st));
+        launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
+                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
+        J2L_HIP(hipEventRecord(ev.e[5], st));
+    }
 t1_again:
     J2L_HIP(hipEventRecord(ev.e[1], st));
     if (use_lanes) {
@@ -463,6 +486,10 @@ t1_again:
             use_lanes = false;
             goto t1_again;
         }
+    }
+    if (probe) {
+        const float t_wave = ev.ms(4, 5), t_lane = ev.ms(1, 2);
+        tuned_[bucket] = use_lanes && t_lane < t_wave ? 2 : 1;
     }
     last_t1_lanes = use_lanes;
     const uint32_t* totals = (const uint32_t*)(P + h_tot);

@@ -2,6 +2,7 @@
 // whole encode/decode batches (one image = a batch of one).
 #pragma once
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -76,11 +77,13 @@ public:
     const std::vector<uint32_t>& last_segment_offsets() const { return last_seg_offs_; }
     const std::vector<double>& last_lambdas() const { return last_lambdas_; }
     bool keep_debug = false;
-    // Tier-1 encoder choice: 0 = by batch size (lane-per-block coder from
-    // lane_min_blocks code-blocks up), 1 = wave per block, 2 = lane per block.
+    // Tier-1 encoder choice: 0 = tuned per batch-size class (classes from
+    // lane_min_blocks code-blocks up time both coders once), 1 = wave per
+    // block, 2 = lane per block, 3 = lane per block from lane_min_blocks up.
     int t1_mode = 0;
-    uint32_t lane_min_blocks = 10000;
+    uint32_t lane_min_blocks = 8192;
     bool last_t1_lanes = false;
+    std::map<int, int> tuned_;  // size class -> coder (1 wave, 2 lane)
     // Stage timings of the last call (ms, from HIP events): prepare+DWT,
     // Tier-1, PCRD+pack, host Tier-2.
     double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;

@@ -298,7 +298,7 @@ def test_crate_api_roundtrip(gpu):
 def lane_mode(lib):
     lib.jp2g_set_t1_mode(2, 0)
     yield lib
-    lib.jp2g_set_t1_mode(0, 10000)
+    lib.jp2g_set_t1_mode(0, 8192)
 
 
 @pytest.mark.parametrize("name,make,qs", CASES, ids=[c[0] for c in CASES])
@@ -335,17 +335,34 @@ def test_lane_coder_full_page(gpu, ora, lane_mode):
     assert j.encode_u8(px, 85) == ora.encode(px, 85)
 
 
-def test_auto_mode_batch_crosses_threshold(gpu, ora, lib):
-    """A batch above the block threshold runs the lane coder, one below it
-    the wave coder; both give the single-image bytes."""
+def test_threshold_mode_batch_crosses_threshold(gpu, ora, lib):
+    """With a block threshold, a batch above it runs the lane coder, one
+    below it the wave coder; both give the single-image bytes."""
+    pages = list(synth.mixed_pages(5, scale=0.12))
+    singles = [ora.encode(px, q) for px, q in pages]
+    lib.jp2g_set_t1_mode(3, 50)
+    try:
+        assert j.encode_many(pages) == singles
+        assert lib.jp2g_last_t1_lanes() == 1
+        lib.jp2g_set_t1_mode(3, 10 ** 9)
+        assert j.encode_many(pages) == singles
+        assert lib.jp2g_last_t1_lanes() == 0
+    finally:
+        lib.jp2g_set_t1_mode(0, 8192)
+
+
+def test_tuned_mode_probes_then_reuses(gpu, ora, lib):
+    """Tuned mode: the first batch of a size class runs both coders (output
+    unchanged), later batches of the class reuse the choice."""
     pages = list(synth.mixed_pages(5, scale=0.12))
     singles = [ora.encode(px, q) for px, q in pages]
     lib.jp2g_set_t1_mode(0, 50)
     try:
-        assert j.encode_many(pages) == singles
-        assert lib.jp2g_last_t1_lanes() == 1
-        lib.jp2g_set_t1_mode(0, 10 ** 9)
-        assert j.encode_many(pages) == singles
-        assert lib.jp2g_last_t1_lanes() == 0
+        first = j.encode_many(pages)
+        chosen = lib.jp2g_last_t1_lanes()
+        assert first == singles
+        for _ in range(2):
+            assert j.encode_many(pages) == singles
+            assert lib.jp2g_last_t1_lanes() == 0 or chosen == 1
     finally:
-        lib.jp2g_set_t1_mode(0, 10000)
+        lib.jp2g_set_t1_mode(0, 8192)
