@@ -88,6 +88,18 @@ typedef struct {
 /* encode(&Image, &EncodeOptions) (src/encode/mod.rs:84-101). */
 jp2lam_status jp2lam_encode(const jp2lam_image_view* image, const jp2lam_encode_options* opts, uint8_t** out,
                             size_t* out_len);
+/* EncodeMetrics (src/encode/mod.rs:114-122). */
+typedef struct {
+    double psnr_db; /* INFINITY for lossless encodes */
+    double ssim;    /* mean SSIM over 8x8 luma blocks; 1.0 for lossless */
+} jp2lam_encode_metrics;
+/* encode_with_psnr(&Image, &EncodeOptions) (src/encode/mod.rs:124-135):
+ * the encode plus compute_quality_metrics (encode/backend/native/
+ * backend.rs:297-433), evaluated on the device from the encode's own
+ * coefficients.  Gray and sRGB only for quality < 100 (ENCODE_FAILED
+ * otherwise, as the reference); on failure nothing is returned. */
+jp2lam_status jp2lam_encode_with_psnr(const jp2lam_image_view* image, const jp2lam_encode_options* opts, uint8_t** out,
+                                      size_t* out_len, jp2lam_encode_metrics* metrics);
 /* Image::from_gray_bytes / from_rgb_bytes (src/model.rs:87-140) + encode:
  * interleaved u8 pixels, channels 1 (gray) or 3 (sRGB). */
 jp2lam_status jp2lam_encode_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,

@@ -3,8 +3,9 @@
 A Python mirror of the jp2lam crate API (src/lib.rs:19-29) over the C ABI
 in ``include/jp2lam_amd.h``: :func:`encode`, :func:`decode_jp2`,
 :func:`inspect_jp2`, :class:`BatchEncoder` / :class:`BatchDecoder`,
-:class:`EncodeOptions` with quality 0..=100, :class:`Image` with planar i32
-components and the :class:`Jp2LamError` family.  Every codec call runs the
+:func:`encode_with_psnr`, :class:`EncodeOptions` with quality 0..=100,
+:class:`Image` with planar i32 components and the :class:`Jp2LamError`
+family.  Every codec call runs the
 HIP kernels in ``libjp2lam_amd.so``; there is no CPU fallback.
 
 Extensions beyond the crate API: :func:`encode_u8` / :func:`decode_u8` for
@@ -16,7 +17,7 @@ from __future__ import annotations
 import ctypes as C
 import enum
 from dataclasses import dataclass, field
-from typing import Iterable, List, Optional, Sequence
+from typing import Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -25,7 +26,7 @@ from . import _lib
 __all__ = [
     "ColorSpace", "OutputFormat", "Preset", "EncodeOptions", "Component", "Image", "DecodeMetadata",
     "Jp2LamError", "InvalidInput", "EncodeFailed", "DecodeFailed", "UnsupportedFeature", "IoError",
-    "encode", "encode_to_writer", "decode_jp2", "decode_from_reader", "inspect_jp2", "BatchEncoder",
+    "encode", "encode_to_writer", "encode_with_psnr", "EncodeMetrics", "decode_jp2", "decode_from_reader", "inspect_jp2", "BatchEncoder",
     "BatchDecoder", "encode_batch", "decode_batch", "encode_u8", "decode_u8", "encode_many", "decode_many",
     "Server", "Served",
 ]
@@ -236,6 +237,28 @@ def encode(image: Image, options: Optional[EncodeOptions] = None) -> bytes:
     o = _opts(options)
     _check(lib.jp2lam_encode(C.byref(v.view), C.byref(o), C.byref(out), C.byref(n)))
     return _take_bytes(out, n.value)
+
+
+@dataclass(frozen=True)
+class EncodeMetrics:
+    """EncodeMetrics (src/encode/mod.rs:114-122): PSNR in dB (inf for
+    lossless) and mean SSIM over 8x8 luma blocks (1.0 for lossless)."""
+    psnr_db: float
+    ssim: float
+
+
+def encode_with_psnr(image: Image, options: Optional[EncodeOptions] = None) -> Tuple[bytes, EncodeMetrics]:
+    """encode_with_psnr (src/encode/mod.rs:124-135): the codestream and the
+    reference's internal reconstruction metrics (backend.rs:297-433),
+    computed on the device from the encode's own coefficients."""
+    lib = _lib.load()
+    v = _View(image)
+    out = _lib.u8p()
+    n = C.c_size_t()
+    o = _opts(options)
+    m = _lib.MetricsC()
+    _check(lib.jp2lam_encode_with_psnr(C.byref(v.view), C.byref(o), C.byref(out), C.byref(n), C.byref(m)))
+    return _take_bytes(out, n.value), EncodeMetrics(m.psnr_db, m.ssim)
 
 
 def encode_to_writer(image: Image, options: Optional[EncodeOptions], writer) -> None:

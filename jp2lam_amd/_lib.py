@@ -53,6 +53,10 @@ class DecodeItemC(C.Structure):
                 ("on_device", C.c_uint8), ("pad", C.c_uint8 * 3), ("status", C.c_int32)]
 
 
+class MetricsC(C.Structure):
+    _fields_ = [("psnr_db", C.c_double), ("ssim", C.c_double)]
+
+
 class ServedC(C.Structure):
     _fields_ = [("status", C.c_int32), ("data", u8p), ("len", C.c_size_t), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("channels", C.c_uint32), ("batch_size", C.c_uint32),
@@ -62,6 +66,8 @@ class ServedC(C.Structure):
 # (name, restype, argtypes) for every symbol the header declares
 SIGNATURES = [
     ("jp2lam_encode", C.c_int, [C.POINTER(ImageViewC), C.POINTER(EncodeOptionsC), C.POINTER(u8p), C.POINTER(C.c_size_t)]),
+    ("jp2lam_encode_with_psnr", C.c_int, [C.POINTER(ImageViewC), C.POINTER(EncodeOptionsC), C.POINTER(u8p),
+                                          C.POINTER(C.c_size_t), C.POINTER(MetricsC)]),
     ("jp2lam_encode_u8", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(EncodeOptionsC),
                                    C.POINTER(u8p), C.POINTER(C.c_size_t)]),
     ("jp2lam_decode_jp2", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(ImageC)]),

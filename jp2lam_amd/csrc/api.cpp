@@ -95,7 +95,7 @@ DecOut decode_one(const DecItem& it) {
 }
 
 jp2lam_status encode_view(const jp2lam_image_view* image, const jp2lam_encode_options* opts, uint8_t** out,
-                          size_t* out_len) {
+                          size_t* out_len, jp2lam_encode_metrics* metrics = nullptr) {
     if (!opts || !out || !out_len) return fail(INVALID_INPUT, "null argument");
     std::vector<int32_t> planar;
     validate_view(image, planar);
@@ -107,10 +107,15 @@ jp2lam_status encode_view(const jp2lam_image_view* image, const jp2lam_encode_op
     it.format = opts->format;
     it.src_kind = SRC_HOST_I32;
     it.src = planar.data();
+    it.metrics = metrics != nullptr;
     EncOut r = encode_one(it);
     if (r.status != OK) return fail(r.status, r.message);
     *out = copy_out(r.bytes);
     *out_len = r.bytes.size();
+    if (metrics) {
+        metrics->psnr_db = r.psnr_db;
+        metrics->ssim = r.ssim;
+    }
     return JP2LAM_OK;
 }
 
@@ -172,6 +177,14 @@ extern "C" {
 jp2lam_status jp2lam_encode(const jp2lam_image_view* image, const jp2lam_encode_options* opts, uint8_t** out,
                             size_t* out_len) {
     return guarded([&] { return encode_view(image, opts, out, out_len); });
+}
+
+jp2lam_status jp2lam_encode_with_psnr(const jp2lam_image_view* image, const jp2lam_encode_options* opts, uint8_t** out,
+                                     size_t* out_len, jp2lam_encode_metrics* metrics) {
+    return guarded([&] {
+        if (!metrics) return fail(INVALID_INPUT, "null argument");
+        return encode_view(image, opts, out, out_len, metrics);
+    });
 }
 
 jp2lam_status jp2lam_encode_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,

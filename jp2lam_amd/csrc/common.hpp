@@ -122,6 +122,28 @@ struct PixSeg {
     uint32_t blk_start, pad;
 };
 
+// Quantisation of a 9/7 coefficient: trunc(v / step) with IEEE division,
+// non-finite -> 0 (simd/scalar.rs:207-213).
+__device__ __forceinline__ int32_t quantize(float v, float step) {
+    if (!(step > 0.0f) || !isfinite(v)) return 0;
+    float q = truncf(__fdiv_rn(v, step));
+    if (q >= 2147483648.0f) return 0x7fffffff;
+    if (q < -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)q;
+}
+
+// Quality-metric launch segment: one image.  8x8 cells tile the image
+// (partial cells on the right and bottom edges count for PSNR only).
+struct MetricSeg {
+    uint64_t rec[3];    // f32 reconstruction planes (level-shifted; Y, Cb, Cr for sRGB)
+    uint64_t orig;      // encoder input samples: u8 interleaved or i32 planar
+    uint32_t width, height;
+    uint32_t cells_x, cells_y;
+    uint32_t cell_start;
+    int32_t ncomp, orig_i32;
+    uint32_t pad;
+};
+
 // ---- kernel launchers (kernels_*.hip) ----
 void launch_fwd_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles, hipStream_t st);
 void launch_inv_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles, hipStream_t st);
@@ -139,6 +161,10 @@ void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const 
 void launch_pcrd(int n, bool lossless, int ngroups, const BlockResult* res, const uint32_t* snaps,
                  const uint16_t* newsigs, const PcrdJob* jobs, HullPt* hull, uint8_t* hull_n, unsigned long long* gmax,
                  const double* raw_floor, double* lambda, Selection* sel, hipStream_t st);
+void launch_metric_dequant(const BlockJob* jobs, const BlockResult* res, const Selection* sel, const uint32_t* list,
+                           int n, hipStream_t st);
+void launch_metric_cells(const MetricSeg* segs, int nseg, uint32_t ncells, double* cells, hipStream_t st);
+void launch_metric_reduce(const MetricSeg* segs, int nseg, const double* cells, double* out, hipStream_t st);
 void launch_pack(int n, int nimg, const uint32_t* img_first, const uint32_t* blk_img, const uint64_t* pack_base,
                  const BlockJob* jobs, const Selection* sel, uint32_t* offs, uint32_t* totals, const uint8_t* slots,
                  uint8_t* packed, hipStream_t st, bool offsets_only);

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 namespace j2l {
@@ -123,6 +124,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         uint32_t blk_first = 0, blk_count = 0;
         size_t pack_cap = 0;
         uint32_t group0 = 0;
+        int mseg = -1;  // index of its quality-metric segment
     };
     std::vector<Img> imgs;
     for (size_t i = 0; i < items.size(); i++) {
@@ -150,6 +152,9 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                 }
             }
             resolution_ladder(it.width, it.height, im.plan.levels, im.rw, im.rh);
+            // backend.rs:371-421: metrics cover gray and sRGB only
+            if (it.metrics && !im.plan.lossless && it.colorspace != GRAY && it.colorspace != SRGB)
+                throw Error(ENCODE_FAILED, "quality metrics not implemented for this colorspace configuration");
             imgs.push_back(std::move(im));
         } catch (const Error& e) {
             outs[i].status = e.status;
@@ -348,6 +353,64 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         o_pseg_k[kind] = L.add(sizeof(PixSeg) * std::max<size_t>(pseg_k[kind].size(), 1));
         for (int k = 0; k < maxL; k++) o_fseg_k[kind][k] = L.add(sizeof(DwtSeg) * std::max<size_t>(fseg_k[kind][k].size(), 1));
     }
+    // encode_with_psnr (backend.rs:305-433): the lossy images' blocks are
+    // dequantised as truncated in place, inverted into their x planes
+    // (free after the forward DWT) and compared cell by cell with the input.
+    std::vector<uint32_t> m_list;
+    std::vector<DwtSeg> m_iseg[kMaxLevels];
+    uint32_t m_itiles[kMaxLevels] = {};
+    std::vector<MetricSeg> m_seg;
+    std::vector<char> m_orig_dev;
+    uint32_t m_cells = 0;
+    for (auto& im : imgs) {
+        if (!items[im.idx].metrics || im.plan.lossless) continue;
+        const Plan& p = im.plan;
+        const int Lv = p.levels;
+        for (uint32_t b = im.blk_first; b < im.blk_first + im.blk_count; b++) m_list.push_back(b);
+        for (int k = 0; k < Lv; k++) {
+            const int r = k + 1;
+            for (int c = 0; c < p.ncomp; c++) {
+                DwtSeg s{};
+                s.in = k == 0 ? im.p_off[c] : im.ll_off[c][(k - 1) & 1];
+                s.in_stride = k == 0 ? (int)p.width : (int)im.rw[r - 1];
+                s.plane = im.p_off[c];
+                s.pstride = (int)p.width;
+                const bool last = k == Lv - 1;
+                s.out = last ? im.x_off[c] : im.ll_off[c][k & 1];
+                s.out_stride = last ? (int)p.width : (int)im.rw[r];
+                s.rw = (int)im.rw[r];
+                s.rh = (int)im.rh[r];
+                s.tiles_x = (s.rw + kTile - 1) / kTile;
+                s.tile_start = m_itiles[k];
+                m_itiles[k] += (uint32_t)s.tiles_x * (uint32_t)((s.rh + kTile - 1) / kTile);
+                m_iseg[k].push_back(s);
+            }
+        }
+        MetricSeg ms{};
+        for (int c = 0; c < 3; c++) ms.rec[c] = Lv ? im.x_off[std::min(c, p.ncomp - 1)] : im.p_off[std::min(c, p.ncomp - 1)];
+        const bool dev_src = im.kind == SRC_DEV_U8;
+        ms.orig = dev_src ? (uint64_t)(uintptr_t)im.src : im.in_off;
+        ms.width = p.width;
+        ms.height = p.height;
+        ms.cells_x = (p.width + 7) / 8;
+        ms.cells_y = (p.height + 7) / 8;
+        ms.cell_start = m_cells;
+        ms.ncomp = p.ncomp;
+        ms.orig_i32 = im.kind == SRC_HOST_I32;
+        m_cells += ms.cells_x * ms.cells_y;
+        im.mseg = (int)m_seg.size();
+        m_seg.push_back(ms);
+        m_orig_dev.push_back(dev_src);
+    }
+    size_t o_mlist = 0, o_mseg = 0, o_mcells = 0, o_mout = 0, o_miseg[kMaxLevels] = {};
+    if (!m_seg.empty()) {
+        o_mlist = L.add(4 * m_list.size());
+        for (int k = 0; k < kMaxLevels; k++)
+            if (!m_iseg[k].empty()) o_miseg[k] = L.add(sizeof(DwtSeg) * m_iseg[k].size());
+        o_mseg = L.add(sizeof(MetricSeg) * m_seg.size());
+        o_mcells = L.add(16 * (size_t)m_cells);
+        o_mout = L.add(16 * m_seg.size());
+    }
 
     dev_.reserve(L.used);
     char* D = dev_.base;
@@ -361,6 +424,16 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                 s.plane = A(s.plane);
                 s.out = A(s.out);
             }
+    }
+    for (int k = 0; k < kMaxLevels; k++)
+        for (auto& s : m_iseg[k]) {
+            s.in = A(s.in);
+            s.plane = A(s.plane);
+            s.out = A(s.out);
+        }
+    for (size_t q = 0; q < m_seg.size(); q++) {
+        for (auto& r : m_seg[q].rec) r = A(r);
+        if (!m_orig_dev[q]) m_seg[q].orig = A(m_seg[q].orig);
     }
     {
         size_t q[2] = {0, 0};
@@ -497,6 +570,23 @@ t1_again:
     std::vector<BlockResult> resv(res, res + nb);
     std::vector<uint32_t> totv(totals, totals + imgs.size());
     last_lambdas_.assign((const double*)(P + h_lam), (const double*)(P + h_lam) + ngroups);
+    std::vector<double> m_out(2 * m_seg.size());
+    if (!m_seg.empty()) {
+        up(o_mlist, m_list.data(), 4 * m_list.size());
+        for (int k = 0; k < kMaxLevels; k++) up(o_miseg[k], m_iseg[k].data(), sizeof(DwtSeg) * m_iseg[k].size());
+        up(o_mseg, m_seg.data(), sizeof(MetricSeg) * m_seg.size());
+        J2L_HIP(hipEventRecord(ev.e[4], st));
+        launch_metric_dequant((const BlockJob*)(D + o_jobs), (const BlockResult*)(D + o_res), (const Selection*)(D + o_sel),
+                              (const uint32_t*)(D + o_mlist), (int)m_list.size(), st);
+        for (int k = 0; k < kMaxLevels; k++)
+            if (!m_iseg[k].empty())
+                launch_inv_level(true, (const DwtSeg*)(D + o_miseg[k]), (int)m_iseg[k].size(), m_itiles[k], st);
+        launch_metric_cells((const MetricSeg*)(D + o_mseg), (int)m_seg.size(), m_cells, (double*)(D + o_mcells), st);
+        launch_metric_reduce((const MetricSeg*)(D + o_mseg), (int)m_seg.size(), (const double*)(D + o_mcells),
+                             (double*)(D + o_mout), st);
+        J2L_HIP(hipEventRecord(ev.e[5], st));
+        J2L_HIP(hipMemcpyAsync(m_out.data(), D + o_mout, 8 * m_out.size(), hipMemcpyDeviceToHost, st));
+    }
     size_t packed_bytes = 0;
     for (size_t k = 0; k < imgs.size(); k++) packed_bytes += totv[k];
     std::vector<uint8_t> packed_host(packed_bytes);
@@ -519,6 +609,7 @@ t1_again:
         for (int b = 0; b < nb; b++) last_seg_offs_[b] = jobs[b].out_off;
     }
     J2L_HIP(hipStreamSynchronize(st));
+    t_metrics_ms = m_seg.empty() ? 0.0 : ev.ms(4, 5);
 
     // ---- host Tier-2 + container --------------------------------------------------------
     auto th0 = std::chrono::steady_clock::now();
@@ -601,6 +692,20 @@ t1_again:
             out.bytes.insert(out.bytes.end(), cs.begin(), cs.end());
         } else {
             out.bytes = std::move(cs);
+        }
+        if (items[im.idx].metrics) {
+            if (im.mseg < 0) {
+                // backend.rs:308-314: lossless encodes
+                out.psnr_db = HUGE_VAL;
+                out.ssim = 1.0;
+            } else {
+                // backend.rs:423-430
+                const MetricSeg& ms = m_seg[im.mseg];
+                const double mse = m_out[2 * im.mseg] / (double)((size_t)ms.width * ms.height * ms.ncomp);
+                out.psnr_db = mse < 1e-10 ? 100.0 : 20.0 * std::log10(255.0 / std::sqrt(mse));
+                const size_t full = (size_t)(ms.width / 8) * (ms.height / 8);
+                out.ssim = full ? m_out[2 * im.mseg + 1] / (double)full : 1.0;
+            }
         }
     }
     t_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();

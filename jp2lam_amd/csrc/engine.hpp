@@ -20,12 +20,14 @@ struct EncItem {
     int format = 0;         // 0 JP2, 1 J2K
     int src_kind = SRC_HOST_U8;
     const void* src = nullptr;  // u8 interleaved (gray/RGB) or i32 planar (ncomp planes)
+    bool metrics = false;       // also compute encode_with_psnr's PSNR / SSIM
 };
 
 struct EncOut {
     Status status = OK;
     std::string message;
     std::vector<uint8_t> bytes;
+    double psnr_db = 0.0, ssim = 0.0;  // EncItem::metrics only
 };
 
 enum DecodeTarget : int { DST_HOST_I32 = 0, DST_HOST_U8 = 1, DST_DEV_U8 = 2 };
@@ -87,6 +89,7 @@ public:
     // Stage timings of the last call (ms, from HIP events): prepare+DWT,
     // Tier-1, PCRD+pack, host Tier-2.
     double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;
+    double t_metrics_ms = 0;  // encode_with_psnr reconstruction + reductions
     hipStream_t stream() const { return stream_; }
     std::mutex mu;
 

@@ -896,16 +896,6 @@ __device__ __forceinline__ uint32_t mqe_flush(MqE& m) {
 
 }  // namespace
 
-// Quantisation of a 9/7 coefficient: trunc(v / step) with IEEE division,
-// non-finite -> 0 (simd/scalar.rs:207-213).
-__device__ __forceinline__ int32_t quantize(float v, float step) {
-    if (!(step > 0.0f) || !isfinite(v)) return 0;
-    float q = truncf(__fdiv_rn(v, step));
-    if (q >= 2147483648.0f) return 0x7fffffff;
-    if (q < -2147483648.0f) return (int32_t)0x80000000;
-    return (int32_t)q;
-}
-
 // Load one code-block, quantise it (9/7) and form the block statistics of
 // t1.rs:215-258: lane x gets column x as row magnitudes and a negative mask.
 // Returns mbp (0 for an all-zero block).

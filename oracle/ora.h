@@ -150,6 +150,12 @@ int ora_pcrd_apply(ora_encoding *enc, const ora_plan *plan);
 int ora_encode(const int32_t *const *planes, uint32_t w, uint32_t h, int ncomp, int colorspace,
                int quality, int format, ora_encoding *enc);
 void ora_encoding_free(ora_encoding *enc);
+/* encode_with_psnr metrics (backend.rs:297-433, :909-959) */
+int ora_metric_recon(const int32_t *const *planes, const ora_encoding *enc, float *recon);
+void ora_metrics_from_recon(const int32_t *const *orig, const float *recon, uint32_t w, uint32_t h, int ncomp,
+                            double *psnr, double *ssim);
+int ora_quality_metrics(const int32_t *const *planes, int colorspace, const ora_encoding *enc, double *psnr,
+                        double *ssim, char *err);
 int ora_decode(const uint8_t *bytes, size_t len, uint32_t *w, uint32_t *h, int *ncomp,
                int *colorspace, int32_t **planes, char *err);
 

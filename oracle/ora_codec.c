@@ -3,6 +3,7 @@
  * Restates encode/mod.rs:84-101, encode/context.rs:27-70,
  * encode/backend/native/backend.rs:52-242, :436-517, layout.rs:37-173,
  * decode/mod.rs:56-129, decode/t1.rs:32-73 and decode/reconstruct.rs. */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -193,9 +194,11 @@ void ora_encoding_free(ora_encoding *enc) {
     memset(enc, 0, sizeof(*enc));
 }
 
-/* decode/mod.rs:56-129 + decode/t1.rs:32-73 + decode/reconstruct.rs:24-302 */
-int ora_decode(const uint8_t *bytes, size_t len, uint32_t *ow, uint32_t *oh, int *onc, int *ocs,
-               int32_t **oplanes, char *err) {
+/* decode/mod.rs:56-129 + decode/t1.rs:32-73 + decode/reconstruct.rs:24-302.
+ * recon (optional, 9/7 only): the f32 component planes after the inverse
+ * DWT, before the inverse colour transform and finalisation. */
+static int decode_impl(const uint8_t *bytes, size_t len, uint32_t *ow, uint32_t *oh, int *onc, int *ocs,
+                       int32_t **oplanes, char *err, float **recon) {
     ora_jp2_header jh;
     const uint8_t *cs;
     size_t cslen;
@@ -312,6 +315,10 @@ int ora_decode(const uint8_t *bytes, size_t len, uint32_t *ow, uint32_t *oh, int
             }
             ora_inv97(fc, w, h, L);
         }
+        if (recon) {
+            *recon = malloc(sizeof(float) * n * (size_t)nc);
+            memcpy(*recon, f, sizeof(float) * n * (size_t)nc);
+        }
         if (nc == 3 && hd.mct) ora_inverse_ict(f, f + n, f + 2 * n, n);
         for (int c = 0; c < nc; c++) ora_finalize_f32(f + (size_t)c * n, n, planes + (size_t)c * n);
         free(f);
@@ -324,6 +331,146 @@ int ora_decode(const uint8_t *bytes, size_t len, uint32_t *ow, uint32_t *oh, int
     *ocs = jh.colorspace;
     *oplanes = planes;
     return ORA_OK;
+}
+
+int ora_decode(const uint8_t *bytes, size_t len, uint32_t *ow, uint32_t *oh, int *onc, int *ocs,
+               int32_t **oplanes, char *err) {
+    return decode_impl(bytes, len, ow, oh, onc, ocs, oplanes, err, NULL);
+}
+
+/* ---- encode_with_psnr metrics (encode/mod.rs:114-135) ------------------------ */
+
+/* backend.rs:626-639: midpoint reconstruction of a coefficient whose block
+ * kept its passes down to bit-plane last_bp (-1: no pass kept). */
+static float dequantize_truncated(int32_t q, int last_bp, float step) {
+    if (last_bp < 0) return 0.0f;
+    uint32_t aq = q < 0 ? (uint32_t)(-(int64_t)q) : (uint32_t)q;
+    uint32_t at = (aq >> last_bp) << last_bp;
+    if (at == 0) return 0.0f;
+    float sign = q >= 0 ? 1.0f : -1.0f;
+    return (sign * ((float)at + 0.5f)) * step;
+}
+
+/* backend.rs:331-368: the f32 component planes the encoder's PCRD choice
+ * implies, after the inverse 9/7 (level-shifted domain).  planes are the
+ * encoder's input samples (RGB for sRGB); recon holds ncomp planes. */
+int ora_metric_recon(const int32_t *const *planes, const ora_encoding *enc, float *recon) {
+    const ora_plan *p = &enc->plan;
+    if (p->lossless) return ORA_ENCODE_FAILED;
+    size_t n = (size_t)p->width * p->height;
+    int32_t *coef = malloc(sizeof(int32_t) * n);
+    for (int c = 0; c < p->ncomp; c++) {
+        float *rc = recon + (size_t)c * n;
+        component_coefficients(p, planes, c, coef);
+        memset(rc, 0, sizeof(float) * n);
+        for (int i = 0; i < enc->nblocks; i++) {
+            const ora_block *b = &enc->blocks[i];
+            if (b->comp != c) continue;
+            float step = 0.0f;
+            for (int k = 0; k < p->nbands; k++)
+                if (p->sq[k].resolution == b->res && p->sq[k].band == b->band)
+                    step = ora_quant_step_f32(8, b->band, p->sq[k].exponent, p->sq[k].mantissa);
+            int last_bp = b->npasses ? b->passes[b->npasses - 1].bitplane : -1;
+            for (int y = b->y0; y < b->y1; y++)
+                for (int x = b->x0; x < b->x1; x++) {
+                    size_t k = (size_t)y * p->width + (size_t)x;
+                    rc[k] = dequantize_truncated(coef[k], last_bp, step);
+                }
+        }
+        ora_inv97(rc, p->width, p->height, p->levels);
+    }
+    free(coef);
+    return ORA_OK;
+}
+
+/* backend.rs:909-959: mean SSIM over the non-overlapping 8x8 blocks. */
+static double mssim_8x8(const float *orig, const float *rec, size_t w, size_t h) {
+    const double C1 = 6.5025, C2 = 58.5225;
+    size_t rows = h / 8, cols = w / 8;
+    if (rows == 0 || cols == 0) return 1.0;
+    double total = 0.0;
+    for (size_t br = 0; br < rows; br++)
+        for (size_t bc = 0; bc < cols; bc++) {
+            double sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
+            for (size_t dy = 0; dy < 8; dy++)
+                for (size_t dx = 0; dx < 8; dx++) {
+                    size_t i = (br * 8 + dy) * w + bc * 8 + dx;
+                    double x = orig[i], y = rec[i];
+                    sx += x;
+                    sy += y;
+                    sxx += x * x;
+                    syy += y * y;
+                    sxy += x * y;
+                }
+            double ux = sx / 64.0, uy = sy / 64.0;
+            double vx = sxx / 64.0 - ux * ux, vy = syy / 64.0 - uy * uy;
+            if (vx < 0.0) vx = 0.0;
+            if (vy < 0.0) vy = 0.0;
+            double cxy = sxy / 64.0 - ux * uy;
+            total += ((2.0 * ux * uy + C1) * (2.0 * cxy + C2)) / ((ux * ux + uy * uy + C1) * (vx + vy + C2));
+        }
+    return total / (double)(rows * cols);
+}
+
+static float clamp255(float v) { return v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v); }
+
+/* backend.rs:370-432: PSNR over every sample and SSIM over BT.601 luma of
+ * the reconstruction recon (gray: 1 plane; sRGB: Y, Cb, Cr planes). */
+void ora_metrics_from_recon(const int32_t *const *orig, const float *recon, uint32_t w, uint32_t h, int ncomp,
+                            double *psnr, double *ssim) {
+    size_t n = (size_t)w * h;
+    float *dl = malloc(sizeof(float) * n), *ol = malloc(sizeof(float) * n);
+    double sse = 0.0;
+    size_t samples;
+    if (ncomp == 1) {
+        samples = n;
+        for (size_t i = 0; i < n; i++) {
+            float d = clamp255(recon[i] + 128.0f);
+            float diff = d - (float)orig[0][i];
+            sse += (double)(diff * diff);
+            dl[i] = d;
+            ol[i] = (float)orig[0][i];
+        }
+    } else {
+        samples = n * 3;
+        const float *yc = recon, *cb = recon + n, *cr = recon + 2 * n;
+        for (size_t i = 0; i < n; i++) {
+            float r = clamp255((yc[i] + 1.402f * cr[i]) + 128.0f);
+            float g = clamp255(((yc[i] - 0.34413f * cb[i]) - 0.71414f * cr[i]) + 128.0f);
+            float b = clamp255((yc[i] + 1.772f * cb[i]) + 128.0f);
+            float dr = r - (float)orig[0][i], dg = g - (float)orig[1][i], db = b - (float)orig[2][i];
+            sse += (double)((dr * dr + dg * dg) + db * db);
+            dl[i] = (0.299f * r + 0.587f * g) + 0.114f * b;
+            ol[i] = (0.299f * (float)orig[0][i] + 0.587f * (float)orig[1][i]) + 0.114f * (float)orig[2][i];
+        }
+    }
+    double mse = sse / (double)samples;
+    *psnr = mse < 1e-10 ? 100.0 : 20.0 * log10(255.0 / sqrt(mse));
+    *ssim = mssim_8x8(ol, dl, w, h);
+    free(dl);
+    free(ol);
+}
+
+/* backend.rs:305-433 (compute_quality_metrics) after a successful encode
+ * of planes (the caller's image components, colour space as given). */
+int ora_quality_metrics(const int32_t *const *planes, int colorspace, const ora_encoding *enc, double *psnr,
+                        double *ssim, char *err) {
+    const ora_plan *p = &enc->plan;
+    if (p->quality >= 100) {
+        *psnr = INFINITY;
+        *ssim = 1.0;
+        return ORA_OK;
+    }
+    if (!(colorspace == ORA_GRAY || (colorspace == ORA_SRGB && p->ncomp == 3 && p->use_mct))) {
+        snprintf(err, 256, "quality metrics not implemented for this colorspace configuration");
+        return ORA_ENCODE_FAILED;
+    }
+    size_t n = (size_t)p->width * p->height;
+    float *recon = malloc(sizeof(float) * n * (size_t)p->ncomp);
+    int rc = ora_metric_recon(planes, enc, recon);
+    if (!rc) ora_metrics_from_recon(planes, recon, p->width, p->height, p->ncomp, psnr, ssim);
+    free(recon);
+    return rc;
 }
 
 /* ---- flat API for ctypes ---------------------------------------------------- */
@@ -488,4 +635,49 @@ void ora_mq_decode_stream(const uint8_t *data, int len, const uint8_t *ctx, int 
 void ora_tables(uint16_t *qe, uint8_t *mps, uint8_t *nmps, uint8_t *nlps, uint8_t *zc, uint8_t *sc, uint8_t *spb) {
     ora_mq_state_table(qe, mps, nmps, nlps);
     ora_t1_luts((uint8_t(*)[512])zc, sc, spb);
+}
+
+/* encode_with_psnr on interleaved u8 pixels: out2 = {psnr_db, ssim}. */
+int ora_encode_metrics_u8(const uint8_t *pix, uint32_t w, uint32_t h, int channels, int quality, double *out2) {
+    int32_t *planes[3] = {0};
+    u8_to_planes(pix, (size_t)w * h, channels, planes);
+    ora_encoding e;
+    int rc = ora_encode((const int32_t *const *)planes, w, h, channels, channels == 1 ? ORA_GRAY : ORA_SRGB, quality, 0, &e);
+    if (!rc) rc = ora_quality_metrics((const int32_t *const *)planes, channels == 1 ? ORA_GRAY : ORA_SRGB, &e,
+                                      &out2[0], &out2[1], e.err);
+    ora_encoding_free(&e);
+    for (int c = 0; c < channels; c++) free(planes[c]);
+    return rc;
+}
+/* The truncated reconstruction the metrics use (channels planes of w*h). */
+int ora_metric_recon_u8(const uint8_t *pix, uint32_t w, uint32_t h, int channels, int quality, float *out) {
+    int32_t *planes[3] = {0};
+    u8_to_planes(pix, (size_t)w * h, channels, planes);
+    ora_encoding e;
+    int rc = ora_encode((const int32_t *const *)planes, w, h, channels, channels == 1 ? ORA_GRAY : ORA_SRGB, quality, 0, &e);
+    if (!rc) rc = ora_metric_recon((const int32_t *const *)planes, &e, out);
+    ora_encoding_free(&e);
+    for (int c = 0; c < channels; c++) free(planes[c]);
+    return rc;
+}
+/* PSNR/SSIM of a given reconstruction against i32 planes: out2 = {psnr, ssim}. */
+void ora_metrics_flat(const int32_t *orig_flat, const float *recon, uint32_t w, uint32_t h, int ncomp, double *out2) {
+    const int32_t *planes[3] = {0};
+    for (int c = 0; c < ncomp && c < 3; c++) planes[c] = orig_flat + (size_t)c * w * h;
+    ora_metrics_from_recon(planes, recon, w, h, ncomp, &out2[0], &out2[1]);
+}
+/* The decoder's f32 planes after the inverse 9/7 (before colour and
+ * rounding); cap in floats.  Returns the status. */
+int ora_decode_recon_f32(const uint8_t *bytes, size_t len, float *out, size_t cap) {
+    uint32_t w, h;
+    int nc, cs;
+    int32_t *planes = NULL;
+    float *rec = NULL;
+    char err[256];
+    int rc = decode_impl(bytes, len, &w, &h, &nc, &cs, &planes, err, &rec);
+    if (!rc && (!rec || (size_t)w * h * (size_t)nc > cap)) rc = ORA_IO;
+    if (!rc) memcpy(out, rec, sizeof(float) * (size_t)w * h * (size_t)nc);
+    free(planes);
+    free(rec);
+    return rc;
 }

@@ -66,6 +66,10 @@ class Oracle:
                                                         C.POINTER(C.c_uint8), C.POINTER(C.c_uint16)]),
             "ora_get_norm_97": (C.c_double, [C.c_uint32, C.c_int]),
             "ora_max_decompositions": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+            "ora_encode_metrics_u8": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, vp]),
+            "ora_metric_recon_u8": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, vp]),
+            "ora_metrics_flat": (None, [vp, vp, C.c_uint32, C.c_uint32, C.c_int, vp]),
+            "ora_decode_recon_f32": (C.c_int, [vp, C.c_size_t, vp, C.c_size_t]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -107,6 +111,45 @@ class Oracle:
             n = w.value * h.value * nc.value
             a = out[:n].copy()
             return a.reshape(h.value, w.value) if nc.value == 1 else a.reshape(h.value, w.value, nc.value)
+
+    # ---- encode_with_psnr metrics ----
+    def encode_metrics(self, px: np.ndarray, quality: int):
+        """(psnr_db, ssim) of encode_with_psnr on u8 pixels."""
+        px = np.ascontiguousarray(px, dtype=np.uint8)
+        h, w = px.shape[:2]
+        ch = 1 if px.ndim == 2 else px.shape[2]
+        out = np.zeros(2, dtype=np.float64)
+        rc = self.L.ora_encode_metrics_u8(px.ctypes.data, w, h, ch, quality, out.ctypes.data)
+        if rc:
+            raise OracleError(rc, "metrics")
+        return float(out[0]), float(out[1])
+
+    def metric_recon(self, px: np.ndarray, quality: int) -> np.ndarray:
+        """The truncated f32 reconstruction the metrics compare (ch, h, w)."""
+        px = np.ascontiguousarray(px, dtype=np.uint8)
+        h, w = px.shape[:2]
+        ch = 1 if px.ndim == 2 else px.shape[2]
+        out = np.zeros((ch, h, w), dtype=np.float32)
+        assert self.L.ora_metric_recon_u8(px.ctypes.data, w, h, ch, quality, out.ctypes.data) == 0
+        return out
+
+    def metrics_of(self, orig: np.ndarray, recon: np.ndarray):
+        """(psnr_db, ssim) of recon (ch, h, w, level-shifted f32) against
+        orig (ch, h, w) samples."""
+        orig = np.ascontiguousarray(orig, dtype=np.int32)
+        recon = np.ascontiguousarray(recon, dtype=np.float32)
+        ch, h, w = orig.shape
+        out = np.zeros(2, dtype=np.float64)
+        self.L.ora_metrics_flat(orig.ctypes.data, recon.ctypes.data, w, h, ch, out.ctypes.data)
+        return float(out[0]), float(out[1])
+
+    def decode_recon(self, data: bytes, shape) -> np.ndarray:
+        """The decoder's f32 planes after the inverse 9/7 (ch, h, w)."""
+        out = np.zeros(shape, dtype=np.float32)
+        rc = self.L.ora_decode_recon_f32(data, len(data), out.ctypes.data, out.size)
+        if rc:
+            raise OracleError(rc, "decode_recon")
+        return out
 
     def coefficients(self, px: np.ndarray, quality: int) -> np.ndarray:
         px = np.ascontiguousarray(px, dtype=np.uint8)

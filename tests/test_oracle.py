@@ -275,3 +275,106 @@ def test_lossless_payload_identical_to_opj_compress(ora, tmp_path):
         return s[i + 2:-2]
 
     assert payload(theirs) == payload(ours)
+
+
+# ---- encode_with_psnr metrics (backend.rs:297-433, :909-959) -------------------
+
+_C1, _C2 = (0.01 * 255) ** 2, (0.03 * 255) ** 2
+
+
+def test_metrics_identical_reconstruction(ora):
+    # mse < 1e-10 -> 100 dB; SSIM of a signal with itself is 1
+    px = synth.photo_page(37, 29, seed=1)[..., 0].astype(np.int32)
+    psnr, ssim = ora.metrics_of(px[None], (px - 128).astype(np.float32)[None])
+    assert psnr == 100.0
+    assert abs(ssim - 1.0) < 1e-15
+
+
+def test_metrics_flat_offset_closed_form(ora):
+    # flat a vs flat a+d: PSNR = 20 log10(255/|d|); per-block SSIM reduces to
+    # the luminance term (2a(a+d)+C1)/(a^2+(a+d)^2+C1); 21x13 has 2x1 full blocks
+    a, d = 90, 7
+    orig = np.full((1, 13, 21), a, np.int32)
+    rec = np.full((1, 13, 21), a + d - 128, np.float32)
+    psnr, ssim = ora.metrics_of(orig, rec)
+    assert abs(psnr - 20 * np.log10(255 / d)) < 1e-12
+    want = (2 * a * (a + d) + _C1) / (a * a + (a + d) ** 2 + _C1)
+    assert abs(ssim - want) < 1e-15
+
+
+def test_metrics_shift_reduces_to_luminance_term(ora):
+    # y = x + d on textured blocks: covariance = variance, so the structure
+    # factor is 1 and each block's SSIM is its luminance factor
+    rng = np.random.default_rng(3)
+    x = rng.integers(20, 200, size=(16, 24)).astype(np.int32)
+    d = 11
+    psnr, ssim = ora.metrics_of(x[None], (x + d - 128).astype(np.float32)[None])
+    assert abs(psnr - 20 * np.log10(255 / d)) < 1e-12
+    vals = []
+    for by in range(2):
+        for bx in range(3):
+            u = x[by * 8:(by + 1) * 8, bx * 8:(bx + 1) * 8].mean()
+            vals.append((2 * u * (u + d) + _C1) / (u * u + (u + d) ** 2 + _C1))
+    assert abs(ssim - np.mean(vals)) < 1e-12
+
+
+def test_metrics_rgb_counts_three_samples_per_pixel(ora):
+    # Y = Cb = Cr = 0 reconstructs mid-grey 128 in every channel exactly
+    rng = np.random.default_rng(4)
+    orig = rng.integers(0, 256, size=(3, 16, 16)).astype(np.int32)
+    psnr, ssim = ora.metrics_of(orig, np.zeros((3, 16, 16), np.float32))
+    mse = ((orig - 128.0) ** 2).sum() / orig.size
+    assert abs(psnr - 20 * np.log10(255 / np.sqrt(mse))) < 1e-10
+    # constant reconstruction: SSIM = luminance factor * C2 / (var_x + C2)
+    luma = 0.299 * orig[0] + 0.587 * orig[1] + 0.114 * orig[2]
+    uy = np.float32(np.float32(0.299) * np.float32(128) + np.float32(0.587) * np.float32(128)) + np.float32(0.114) * np.float32(128)
+    vals = []
+    for by in range(2):
+        for bx in range(2):
+            blk = luma[by * 8:(by + 1) * 8, bx * 8:(bx + 1) * 8]
+            ux, vx = blk.mean(), blk.var()
+            vals.append((2 * ux * uy + _C1) * _C2 / ((ux * ux + float(uy) ** 2 + _C1) * (vx + _C2)))
+    assert abs(ssim - np.mean(vals)) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["gray", "rgb"])
+def test_metric_recon_is_decoder_recon_when_all_passes_kept(ora, kind):
+    # q=99: lambda is 0, every pass is kept, the last pass of every block is
+    # the cleanup at bit-plane 0, so the truncated midpoint reconstruction is
+    # the decoder's (decode/reconstruct.rs) before colour and rounding
+    px = synth.photo_page(70, 53, seed=8) if kind == "rgb" else synth.gradient_page(70, 53, seed=8)
+    data, enc = ora.encode(px, 99, keep=True)
+    for i in range(enc.nblocks):
+        info = enc.info(i)
+        assert info[10] == info[11]
+    rec = ora.metric_recon(px, 99)
+    assert np.array_equal(rec, ora.decode_recon(data, rec.shape))
+
+
+def test_metric_recon_matches_openjpeg_at_full_retention(ora):
+    PIL = pytest.importorskip("PIL.Image")
+    px = synth.gradient_page(96, 80, seed=2)
+    data = ora.encode(px, 99)
+    theirs = np.array(PIL.open(io.BytesIO(data))).astype(int)
+    rec = ora.metric_recon(px, 99)[0]
+    assert np.abs(np.clip(np.round(rec + 128), 0, 255) - theirs).max() <= 1
+
+
+@pytest.mark.parametrize("q", [85, 50, 20])
+def test_metric_psnr_tracks_decoded_psnr(ora, q):
+    # the estimate assumes every coefficient of a block coded down to its
+    # last pass's bit-plane; the real decode is close below that
+    px = synth.gradient_page(128, 96, seed=q)
+    psnr, ssim = ora.encode_metrics(px, q)
+    dec = ora.decode(ora.encode(px, q)).astype(np.float64)
+    mse = ((dec - px) ** 2).mean()
+    real = 20 * np.log10(255 / np.sqrt(mse))
+    # optimistic: passes cut inside a bit-plane still count their plane
+    assert -0.1 < psnr - real < 1.0, (psnr, real)
+    assert 0.0 < ssim <= 1.0
+
+
+def test_metrics_lossless_is_infinite(ora):
+    px = synth.gradient_page(40, 30, seed=1)
+    psnr, ssim = ora.encode_metrics(px, 100)
+    assert psnr == float("inf") and ssim == 1.0
