@@ -100,6 +100,23 @@ typedef struct {
  * otherwise, as the reference); on failure nothing is returned. */
 jp2lam_status jp2lam_encode_with_psnr(const jp2lam_image_view* image, const jp2lam_encode_options* opts, uint8_t** out,
                                       size_t* out_len, jp2lam_encode_metrics* metrics);
+/* ContentStats + ContentClass (src/content.rs:46-105). */
+typedef struct {
+    float bright_share;     /* share of pixels with luma > 224 */
+    float entropy;          /* bits, 256-bin luma histogram */
+    float colorfulness;     /* mean max(r,g,b) - min(r,g,b); 0 for gray */
+    uint32_t content_class; /* 0 Photo, 1 Graphics (ContentStats::classify) */
+    uint32_t quality;       /* ContentClass::quality: 78 Photo, 55 Graphics */
+} jp2lam_content_stats;
+/* analyze_content(&Image) / auto_quality (src/content.rs:119-174): one
+ * histogram pass on the device.  Gray reads component 0, every other
+ * colour space components 0..2 as R, G, B; too few components give zero
+ * statistics (classified Graphics), as the reference. */
+jp2lam_status jp2lam_analyze_content(const jp2lam_image_view* image, jp2lam_content_stats* out);
+/* The same on interleaved u8 pixels (channels 1 gray / 3 RGB), host or
+ * device (on_device) resident. */
+jp2lam_status jp2lam_analyze_content_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                                        int on_device, jp2lam_content_stats* out);
 /* Image::from_gray_bytes / from_rgb_bytes (src/model.rs:87-140) + encode:
  * interleaved u8 pixels, channels 1 (gray) or 3 (sRGB). */
 jp2lam_status jp2lam_encode_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,

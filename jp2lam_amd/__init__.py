@@ -28,7 +28,8 @@ __all__ = [
     "Jp2LamError", "InvalidInput", "EncodeFailed", "DecodeFailed", "UnsupportedFeature", "IoError",
     "encode", "encode_to_writer", "encode_with_psnr", "EncodeMetrics", "decode_jp2", "decode_from_reader", "inspect_jp2", "BatchEncoder",
     "BatchDecoder", "encode_batch", "decode_batch", "encode_u8", "decode_u8", "encode_many", "decode_many",
-    "Server", "Served",
+    "Server", "Served", "ContentClass", "ContentStats", "analyze_content", "auto_quality",
+    "auto_class_and_quality", "analyze_content_u8", "PHOTO_QUALITY", "GRAPHICS_QUALITY",
 ]
 
 
@@ -259,6 +260,72 @@ def encode_with_psnr(image: Image, options: Optional[EncodeOptions] = None) -> T
     m = _lib.MetricsC()
     _check(lib.jp2lam_encode_with_psnr(C.byref(v.view), C.byref(o), C.byref(out), C.byref(n), C.byref(m)))
     return _take_bytes(out, n.value), EncodeMetrics(m.psnr_db, m.ssim)
+
+
+# ---- content-adaptive quality (src/content.rs) --------------------------------------
+PHOTO_QUALITY = 78
+GRAPHICS_QUALITY = 55
+
+
+class ContentClass(enum.Enum):
+    """ContentClass (content.rs:55-78)."""
+    PHOTO = 0
+    GRAPHICS = 1
+
+    def quality(self) -> int:
+        return GRAPHICS_QUALITY if self is ContentClass.GRAPHICS else PHOTO_QUALITY
+
+    def label(self) -> str:
+        return "graphics" if self is ContentClass.GRAPHICS else "photo"
+
+
+@dataclass(frozen=True)
+class ContentStats:
+    """ContentStats (content.rs:80-106): f32 statistics of one histogram pass."""
+    bright_share: float
+    entropy: float
+    colorfulness: float
+
+    def classify(self) -> ContentClass:
+        # content.rs:96-105, compared in f32 like the reference
+        e, b = np.float32(self.entropy), np.float32(self.bright_share)
+        graphics = e < np.float32(3.0) or (b > np.float32(0.45) and e < np.float32(5.0))
+        return ContentClass.GRAPHICS if graphics else ContentClass.PHOTO
+
+
+def _content(st: "_lib.ContentStatsC") -> ContentStats:
+    return ContentStats(st.bright_share, st.entropy, st.colorfulness)
+
+
+def analyze_content(image: Image) -> ContentStats:
+    """analyze (content.rs:119-166): luma histogram and chroma spread of
+    ``image``, one pass on the device."""
+    lib = _lib.load()
+    v = _View(image)
+    st = _lib.ContentStatsC()
+    _check(lib.jp2lam_analyze_content(C.byref(v.view), C.byref(st)))
+    return _content(st)
+
+
+def analyze_content_u8(pixels: np.ndarray) -> ContentStats:
+    """analyze on interleaved u8 pixels, (H, W) gray or (H, W, 3) RGB."""
+    lib = _lib.load()
+    px = np.ascontiguousarray(pixels, dtype=np.uint8)
+    ch = 1 if px.ndim == 2 else px.shape[2]
+    st = _lib.ContentStatsC()
+    _check(lib.jp2lam_analyze_content_u8(px.ctypes.data, px.shape[1], px.shape[0], ch, 0, C.byref(st)))
+    return _content(st)
+
+
+def auto_quality(image: Image) -> int:
+    """auto_quality (content.rs:168-171)."""
+    return analyze_content(image).classify().quality()
+
+
+def auto_class_and_quality(image: Image) -> Tuple[ContentClass, int]:
+    """auto_class_and_quality (content.rs:173-177)."""
+    cls = analyze_content(image).classify()
+    return cls, cls.quality()
 
 
 def encode_to_writer(image: Image, options: Optional[EncodeOptions], writer) -> None:

@@ -57,6 +57,11 @@ class MetricsC(C.Structure):
     _fields_ = [("psnr_db", C.c_double), ("ssim", C.c_double)]
 
 
+class ContentStatsC(C.Structure):
+    _fields_ = [("bright_share", C.c_float), ("entropy", C.c_float), ("colorfulness", C.c_float),
+                ("content_class", C.c_uint32), ("quality", C.c_uint32)]
+
+
 class ServedC(C.Structure):
     _fields_ = [("status", C.c_int32), ("data", u8p), ("len", C.c_size_t), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("channels", C.c_uint32), ("batch_size", C.c_uint32),
@@ -66,6 +71,9 @@ class ServedC(C.Structure):
 # (name, restype, argtypes) for every symbol the header declares
 SIGNATURES = [
     ("jp2lam_encode", C.c_int, [C.POINTER(ImageViewC), C.POINTER(EncodeOptionsC), C.POINTER(u8p), C.POINTER(C.c_size_t)]),
+    ("jp2lam_analyze_content", C.c_int, [C.POINTER(ImageViewC), C.POINTER(ContentStatsC)]),
+    ("jp2lam_analyze_content_u8", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                            C.POINTER(ContentStatsC)]),
     ("jp2lam_encode_with_psnr", C.c_int, [C.POINTER(ImageViewC), C.POINTER(EncodeOptionsC), C.POINTER(u8p),
                                           C.POINTER(C.c_size_t), C.POINTER(MetricsC)]),
     ("jp2lam_encode_u8", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(EncodeOptionsC),

@@ -1,6 +1,8 @@
 // C ABI of jp2lam_amd (include/jp2lam_amd.h).
 #include "../../include/jp2lam_amd.h"
 
+#include <algorithm>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -184,6 +186,53 @@ jp2lam_status jp2lam_encode_with_psnr(const jp2lam_image_view* image, const jp2l
     return guarded([&] {
         if (!metrics) return fail(INVALID_INPUT, "null argument");
         return encode_view(image, opts, out, out_len, metrics);
+    });
+}
+
+jp2lam_status jp2lam_analyze_content(const jp2lam_image_view* image, jp2lam_content_stats* out) {
+    return guarded([&] {
+        if (!image || !out) return fail(INVALID_INPUT, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        const bool gray = image->colorspace == JP2LAM_GRAY;
+        const uint32_t need = gray ? 1 : 3;
+        ContentOut r;
+        if (image->num_components >= need) {
+            size_t n = SIZE_MAX;
+            for (uint32_t c = 0; c < need; c++) {
+                const jp2lam_component& k = image->components[c];
+                if (!k.data && (size_t)k.width * k.height) return fail(INVALID_INPUT, "component has no samples");
+                n = std::min(n, (size_t)k.width * k.height);
+            }
+            std::vector<int32_t> planar(n * need);
+            for (uint32_t c = 0; c < need; c++)
+                if (n) std::memcpy(planar.data() + c * n, image->components[c].data, n * 4);
+            Engine& e = engine();
+            std::lock_guard<std::mutex> lk(e.mu);
+            r = e.analyze_content(planar.data(), SRC_HOST_I32, n, gray);
+        }
+        out->bright_share = r.bright_share;
+        out->entropy = r.entropy;
+        out->colorfulness = r.colorfulness;
+        out->content_class = (uint32_t)r.graphics;
+        out->quality = r.graphics ? 55u : 78u;
+        return JP2LAM_OK;
+    });
+}
+
+jp2lam_status jp2lam_analyze_content_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                                        int on_device, jp2lam_content_stats* out) {
+    return guarded([&] {
+        if (!pixels || !out) return fail(INVALID_INPUT, "null argument");
+        if (channels != 1 && channels != 3) return fail(INVALID_INPUT, "channels must be 1 (gray) or 3 (sRGB)");
+        Engine& e = engine();
+        std::lock_guard<std::mutex> lk(e.mu);
+        ContentOut r = e.analyze_content(pixels, on_device ? SRC_DEV_U8 : SRC_HOST_U8, (size_t)width * height, channels == 1);
+        out->bright_share = r.bright_share;
+        out->entropy = r.entropy;
+        out->colorfulness = r.colorfulness;
+        out->content_class = (uint32_t)r.graphics;
+        out->quality = r.graphics ? 55u : 78u;
+        return JP2LAM_OK;
     });
 }
 

@@ -713,6 +713,47 @@ t1_again:
 }
 
 // ============================================================================
+// Content analysis
+// ============================================================================
+// content.rs:120-174: the histogram pass runs on the device; the statistics
+// are formed on the host from the 257 counters in the reference's f32 order.
+ContentOut Engine::analyze_content(const void* src, int src_kind, size_t n, bool gray) {
+    ContentOut o;
+    if (n == 0) return o;  // content.rs:155-157
+    const int ch = gray ? 1 : 3;
+    Layout L;
+    size_t in_bytes = src_kind == SRC_HOST_U8 ? n * ch : src_kind == SRC_HOST_I32 ? n * ch * 4 : 0;
+    const size_t in_off = in_bytes ? L.add(in_bytes) : 0;
+    const size_t o_hist = L.add(8 * 257);
+    dev_.reserve(L.used);
+    pin_.reserve(8 * 257);
+    char* D = dev_.base;
+    hipStream_t st = stream_;
+    if (in_bytes) J2L_HIP(hipMemcpyAsync(D + in_off, src, in_bytes, hipMemcpyHostToDevice, st));
+    J2L_HIP(hipMemsetAsync(D + o_hist, 0, 8 * 257, st));
+    launch_content_hist(src_kind == SRC_DEV_U8 ? src : (const void*)(D + in_off), n, ch, src_kind == SRC_HOST_I32, gray,
+                        (unsigned long long*)(D + o_hist), st);
+    J2L_HIP(hipMemcpyAsync(pin_.base, D + o_hist, 8 * 257, hipMemcpyDeviceToHost, st));
+    J2L_HIP(hipStreamSynchronize(st));
+    const uint64_t* h = (const uint64_t*)pin_.base;
+    const float total = (float)(uint64_t)n;
+    uint64_t bright = 0;
+    for (int v = 225; v < 256; v++) bright += h[v];  // luma > 224
+    float entropy = 0.0f;
+    for (int v = 0; v < 256; v++)
+        if (h[v]) {
+            const float p = (float)h[v] / total;
+            entropy -= p * std::log2(p);
+        }
+    o.bright_share = (float)bright / total;
+    o.entropy = entropy;
+    o.colorfulness = (float)h[256] / total;
+    // content.rs:96-105
+    o.graphics = entropy < 3.0f || (o.bright_share > 0.45f && entropy < 5.0f);
+    return o;
+}
+
+// ============================================================================
 // Decode
 // ============================================================================
 std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {

@@ -49,6 +49,12 @@ struct DecOut {
     std::vector<uint8_t> pixels;  // DST_HOST_U8: interleaved
 };
 
+// ContentStats and ContentClass (content.rs:80-105).
+struct ContentOut {
+    float bright_share = 0.0f, entropy = 0.0f, colorfulness = 0.0f;
+    int graphics = 1;  // 1 Graphics, 0 Photo (all-zero statistics classify as Graphics)
+};
+
 // Tier-1 block statistics of the last encode (tests and counters).
 struct BlockStat {
     uint8_t comp, res, band, mbp, zbp, cls, npasses, keep;
@@ -70,6 +76,10 @@ public:
     ~Engine();
     std::vector<EncOut> encode(const std::vector<EncItem>& items);
     std::vector<DecOut> decode(const std::vector<DecItem>& items);
+    // analyze (content.rs:120-166) of n samples per component: gray reads one
+    // plane / channel, otherwise R, G, B.  src: host u8 interleaved, host i32
+    // planar (ncomp planes of n) or device u8 interleaved (SourceKind).
+    ContentOut analyze_content(const void* src, int src_kind, size_t n, bool gray);
     // Debug/test access to the last encode: per-block statistics, pass
     // snapshots and the full Tier-1 segments (before truncation).
     const std::vector<BlockStat>& last_blocks() const { return last_blocks_; }

@@ -13,6 +13,8 @@
 // No FMA contraction (the build passes -ffp-contract=off): per-pixel f32
 // values and per-cell f64 sums follow the reference's order, so each cell's
 // SSIM equals the reference's; only the order of the image-wide sums differs.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace j2l {
@@ -157,6 +159,60 @@ __global__ __launch_bounds__(256) void metric_reduce_kernel(const MetricSeg* __r
         out[2 * blockIdx.x] = sa[0];
         out[2 * blockIdx.x + 1] = sb[0];
     }
+}
+
+// Content analysis (content.rs:120-153): the 256-bin luma histogram and the
+// sum of per-pixel chroma spreads max(r,g,b) - min(r,g,b), in one pass.
+// out[0..255] = bins, out[256] = spread sum.  Gray reads the samples
+// clamped to 0..255; the RGB family takes integer BT.601 luma
+// (content.rs:110-117).  A workgroup sees at most ceil(n / (256 G)) * 256
+// samples for G workgroups, so its LDS bins stay far below 2^32 (under 2^21
+// for 65535 x 65535 samples and G = 4096).
+__global__ __launch_bounds__(256) void content_hist_kernel(const void* __restrict__ src, uint64_t n, int channels,
+                                                           int planar_i32, int gray,
+                                                           unsigned long long* __restrict__ out) {
+    __shared__ uint32_t bins[256];
+    __shared__ unsigned long long spread;
+    bins[threadIdx.x] = 0;
+    if (threadIdx.x == 0) spread = 0;
+    __syncthreads();
+    const uint8_t* s8 = (const uint8_t*)src;
+    const int32_t* s32 = (const int32_t*)src;
+    unsigned long long mine = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        int bin;
+        if (gray) {
+            const int v = planar_i32 ? s32[i] : (int)s8[i * channels];
+            bin = min(max(v, 0), 255);
+        } else {
+            int r, g, b;
+            if (planar_i32) {
+                r = s32[i];
+                g = s32[n + i];
+                b = s32[2 * n + i];
+            } else {
+                r = s8[3 * i];
+                g = s8[3 * i + 1];
+                b = s8[3 * i + 2];
+            }
+            const int rc = min(max(r, 0), 255), gc = min(max(g, 0), 255), bc = min(max(b, 0), 255);
+            bin = min(max((77 * rc + 150 * gc + 29 * bc + 128) >> 8, 0), 255);
+            const int mx = min(max(max(max(r, g), b), 0), 255), mn = min(max(min(min(r, g), b), 0), 255);
+            mine += (unsigned long long)(mx - mn);
+        }
+        atomicAdd(&bins[bin], 1u);
+    }
+    if (mine) atomicAdd(&spread, mine);
+    __syncthreads();
+    if (bins[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)bins[threadIdx.x]);
+    if (threadIdx.x == 0 && spread) atomicAdd(&out[256], spread);
+}
+
+void launch_content_hist(const void* src, uint64_t n, int channels, int planar_i32, int gray, unsigned long long* out,
+                         hipStream_t st) {
+    if (!n) return;
+    const uint64_t groups = std::min<uint64_t>(std::max<uint64_t>(n / 4096, 1), 4096);
+    content_hist_kernel<<<(unsigned)groups, 256, 0, st>>>(src, n, channels, planar_i32, gray, out);
 }
 
 void launch_metric_dequant(const BlockJob* jobs, const BlockResult* res, const Selection* sel, const uint32_t* list,

@@ -154,6 +154,9 @@ void ora_encoding_free(ora_encoding *enc);
 int ora_metric_recon(const int32_t *const *planes, const ora_encoding *enc, float *recon);
 void ora_metrics_from_recon(const int32_t *const *orig, const float *recon, uint32_t w, uint32_t h, int ncomp,
                             double *psnr, double *ssim);
+/* content analysis (content.rs:96-174) */
+void ora_analyze_content(const int32_t *const *planes, size_t n, int ncomp, int colorspace, float *out3);
+int ora_classify_content(const float *stats3);
 int ora_quality_metrics(const int32_t *const *planes, int colorspace, const ora_encoding *enc, double *psnr,
                         double *ssim, char *err);
 int ora_decode(const uint8_t *bytes, size_t len, uint32_t *w, uint32_t *h, int *ncomp,

@@ -681,3 +681,68 @@ int ora_decode_recon_f32(const uint8_t *bytes, size_t len, float *out, size_t ca
     free(rec);
     return rc;
 }
+
+/* ---- content analysis (content.rs:119-174) ---------------------------------- */
+
+/* content.rs:110-117: integer BT.601 luma of clamped 8-bit R, G, B */
+static int luma601(int32_t r, int32_t g, int32_t b) {
+    r = r < 0 ? 0 : (r > 255 ? 255 : r);
+    g = g < 0 ? 0 : (g > 255 ? 255 : g);
+    b = b < 0 ? 0 : (b > 255 ? 255 : b);
+    int v = (77 * r + 150 * g + 29 * b + 128) >> 8;
+    return v < 0 ? 0 : (v > 255 ? 255 : v);
+}
+
+/* content.rs:120-166: out3 = {bright_share, entropy, colorfulness}.  Gray
+ * reads component 0; every other colour space reads components 0..2 as
+ * R, G, B.  n = samples per component. */
+void ora_analyze_content(const int32_t *const *planes, size_t n, int ncomp, int colorspace, float *out3) {
+    uint64_t hist[256] = {0}, colorful = 0, count = 0;
+    if (colorspace == ORA_GRAY) {
+        if (ncomp >= 1)
+            for (size_t i = 0; i < n; i++) {
+                int32_t v = planes[0][i];
+                hist[v < 0 ? 0 : (v > 255 ? 255 : v)]++;
+                count++;
+            }
+    } else if (ncomp >= 3) {
+        for (size_t i = 0; i < n; i++) {
+            int32_t r = planes[0][i], g = planes[1][i], b = planes[2][i];
+            hist[luma601(r, g, b)]++;
+            int32_t mx = r > g ? r : g, mn = r < g ? r : g;
+            mx = mx > b ? mx : b;
+            mn = mn < b ? mn : b;
+            mx = mx < 0 ? 0 : (mx > 255 ? 255 : mx);
+            mn = mn < 0 ? 0 : (mn > 255 ? 255 : mn);
+            colorful += (uint64_t)(int64_t)(mx - mn);
+            count++;
+        }
+    }
+    if (count == 0) {
+        out3[0] = out3[1] = out3[2] = 0.0f;
+        return;
+    }
+    float total = (float)count;
+    uint64_t bright = 0;
+    for (int v = 225; v < 256; v++) bright += hist[v];
+    float entropy = 0.0f;
+    for (int v = 0; v < 256; v++)
+        if (hist[v]) {
+            float p = (float)hist[v] / total;
+            entropy -= p * log2f(p);
+        }
+    out3[0] = (float)bright / total;
+    out3[1] = entropy;
+    out3[2] = (float)colorful / total;
+}
+
+/* content.rs:96-105: 1 = Graphics, 0 = Photo */
+int ora_classify_content(const float *stats3) {
+    return stats3[1] < 3.0f || (stats3[0] > 0.45f && stats3[1] < 5.0f);
+}
+
+void ora_analyze_content_flat(const int32_t *planes_flat, size_t n, int ncomp, int colorspace, float *out3) {
+    const int32_t *planes[3] = {0};
+    for (int c = 0; c < ncomp && c < 3; c++) planes[c] = planes_flat + (size_t)c * n;
+    ora_analyze_content(planes, n, ncomp, colorspace, out3);
+}

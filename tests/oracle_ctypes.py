@@ -70,6 +70,8 @@ class Oracle:
             "ora_metric_recon_u8": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, vp]),
             "ora_metrics_flat": (None, [vp, vp, C.c_uint32, C.c_uint32, C.c_int, vp]),
             "ora_decode_recon_f32": (C.c_int, [vp, C.c_size_t, vp, C.c_size_t]),
+            "ora_analyze_content_flat": (None, [vp, C.c_size_t, C.c_int, C.c_int, vp]),
+            "ora_classify_content": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -150,6 +152,16 @@ class Oracle:
         if rc:
             raise OracleError(rc, "decode_recon")
         return out
+
+    # ---- content analysis ----
+    def analyze_content(self, planes: np.ndarray, colorspace: int):
+        """(bright_share, entropy, colorfulness) as f32 and the class
+        (1 graphics, 0 photo) of planes (ncomp, h, w) i32."""
+        planes = np.ascontiguousarray(planes, dtype=np.int32)
+        nc = planes.shape[0]
+        out = np.zeros(3, dtype=np.float32)
+        self.L.ora_analyze_content_flat(planes.ctypes.data, planes[0].size, nc, colorspace, out.ctypes.data)
+        return out, int(self.L.ora_classify_content(out.ctypes.data))
 
     def coefficients(self, px: np.ndarray, quality: int) -> np.ndarray:
         px = np.ascontiguousarray(px, dtype=np.uint8)

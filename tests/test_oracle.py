@@ -378,3 +378,42 @@ def test_metrics_lossless_is_infinite(ora):
     px = synth.gradient_page(40, 30, seed=1)
     psnr, ssim = ora.encode_metrics(px, 100)
     assert psnr == float("inf") and ssim == 1.0
+
+
+# ---- content analysis (content.rs:96-174) ----------------------------------------
+
+
+def test_content_reference_cases(ora):
+    # content.rs:190-235: the reference's own tests
+    px = np.full(40 * 40, 255, np.int32)
+    px[:200] = 0
+    st, cls = ora.analyze_content(px.reshape(1, 40, 40), 0)
+    assert st[0] > 0.45 and cls == 1
+    ramp = np.tile(np.arange(256, dtype=np.int32), (64, 1))
+    st, cls = ora.analyze_content(ramp[None], 0)
+    assert st[1] > 5.0 and cls == 0
+    st, _ = ora.analyze_content(np.full((1, 8, 8), 128, np.int32), 0)
+    assert st[2] == 0.0
+
+
+def test_content_entropy_closed_forms(ora):
+    # uniform over 256 levels -> 8 bits; two equal levels -> 1 bit; constant -> 0
+    ramp = np.tile(np.arange(256, dtype=np.int32), (4, 1))
+    assert ora.analyze_content(ramp[None], 0)[0][1] == 8.0
+    two = np.array([[10, 200] * 32], np.int32)
+    assert ora.analyze_content(two[None], 0)[0][1] == 1.0
+    st, cls = ora.analyze_content(np.full((1, 5, 7), 240, np.int32), 0)
+    assert st[1] == 0.0 and st[0] == 1.0 and cls == 1
+
+
+def test_content_rgb_luma_and_colorfulness(ora):
+    # luma601 of pure primaries: (77*255+128)>>8 = 77, (150*255+128)>>8 = 149,
+    # (29*255+128)>>8 = 29; colorfulness = mean(max - min) with clamping
+    r = np.array([[255, 0, 0, 300]], np.int32)
+    g = np.array([[0, 255, 0, -5]], np.int32)
+    b = np.array([[0, 0, 255, 0]], np.int32)
+    st, _ = ora.analyze_content(np.stack([r, g, b]), 1)
+    assert st[2] == np.float32(255.0)
+    # luma bins 77, 149, 29, and 77 again for the clamped 4th pixel
+    # (255, 0, 0): p = [0.5, 0.25, 0.25] -> 1.5 bits
+    assert st[1] == 1.5 and st[0] == 0.0
