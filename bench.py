@@ -36,6 +36,11 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of the serving leg (Poisson arrivals of page encodes); 0 = skip")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="serving arrival rate, requests/s")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workloads (512x512 q100 J2K, 4000x3000 sRGB q75, mixed-page queue)")
+    ap.add_argument("--mixed-pages", type=int, default=32,
+                    help="pages of the mixed-size queue leg (the full queue is 256)")
+    ap.add_argument("--mixed-batch", type=int, default=16, help="pages per device batch in the mixed-queue leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -236,6 +241,103 @@ def main():
                  "mean_batch": round(float(np.mean([r.batch_size for r in done])), 2),
                  "batches": int(st["batches"]), "results_equal_alone": bool(ok)}
 
+    # ---- the other workloads: 512x512 q100 J2K, 4000x3000 sRGB q75, mixed queue ----
+    configs = None
+    if not args.no_configs:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle_ctypes import Oracle
+
+        ora = Oracle()
+        configs = {}
+
+        def timed(fn, reps):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                r = fn()
+            torch.cuda.synchronize()
+            return r, (time.perf_counter() - t0) * 1000.0 / reps
+
+        # gradient page, lossless J2K: encode only (the reference decodes JP2 only)
+        g = synth.gradient_page(512, 512, seed=1)
+        dg = torch.from_numpy(g).to(dev)
+        enc1, ms1 = timed(lambda: j.encode_many([((dg.data_ptr(), 512, 512, 1), 100)], format=j.OutputFormat.J2K,
+                                                device=True)[0], max(5, args.steps))
+        configs["gradient512_gray_q100_j2k_encode"] = {
+            "ms": round(ms1, 3), "mpx_s": round(512 * 512 / 1e3 / ms1, 2), "codestream_bytes": len(enc1),
+            "parity_vs_oracle": bool(enc1 == ora.encode(g, 100, fmt=1))}
+
+        # photo page, sRGB q75: encode then decode, pixels resident in HBM
+        W3, H3 = 4000, 3000
+        ph = synth.photo_page(W3, H3, seed=3)
+        dph = torch.from_numpy(ph).to(dev)
+        dph_out = torch.empty_like(dph)
+        enc3, e3 = timed(lambda: j.encode_many([((dph.data_ptr(), H3, W3, 3), 75)], device=True)[0], max(3, args.steps // 2))
+        te3 = np.zeros(4, np.float64)
+        lib.jp2g_last_timings(te3.ctypes.data)
+        coder3 = "lane per block" if lib.jp2g_last_t1_lanes() else "wave per block"
+        _, d3 = timed(lambda: j.decode_many([enc3], device_outputs=[(dph_out.data_ptr(), dph_out.numel())])[0],
+                      max(3, args.steps // 2))
+        td3 = np.zeros(4, np.float64)
+        lib.jp2g_last_timings(td3.ctypes.data)
+        ref3 = ora.encode(ph, 75)
+        configs["photo4000x3000_srgb_q75"] = {
+            "encode_ms": round(e3, 3), "decode_ms": round(d3, 3),
+            "mpx_s": round(W3 * H3 / 1e3 / (e3 + d3), 2), "codestream_bytes": len(enc3), "t1_coder": coder3,
+            "stage_ms": {"dwt": round(te3[0], 3), "t1_encode": round(te3[1], 3), "pcrd_pack": round(te3[2], 3),
+                         "t1_decode": round(td3[1], 3), "idwt_finalize": round(td3[0], 3)},
+            "parity_vs_oracle": bool(enc3 == ref3 and np.array_equal(dph_out.cpu().numpy(), ora.decode(ref3)))}
+        del dph, dph_out
+
+        # mixed-size queue: 70 % documents / 30 % photos, q in {50, 75, 85, 100},
+        # encoded and decoded in device batches of --mixed-batch pages
+        if args.mixed_pages > 0:
+            host = list(synth.mixed_pages(args.mixed_pages, seed=4))
+            devp = [torch.from_numpy(p).to(dev) for p, _ in host]
+            items = [((d.data_ptr(), p.shape[0], p.shape[1], 1 if p.ndim == 2 else 3), q)
+                     for d, (p, q) in zip(devp, host)]
+            outs = [torch.empty_like(d) for d in devp]
+            B = max(1, args.mixed_batch)
+            chunks = [list(range(i, min(i + B, len(items)))) for i in range(0, len(items), B)]
+            # sRGB lossless pages are rejected by the reference decoder (its
+            # chroma Mb quirk), so the decode half skips them
+            dec_ids = [k for k in range(len(host)) if not (host[k][0].ndim == 3 and host[k][1] >= 100)]
+            dchunks = [dec_ids[i:i + B] for i in range(0, len(dec_ids), B)]
+            streams = [None] * len(items)
+
+            def enc_all():
+                for ch in chunks:
+                    for k, r in zip(ch, j.encode_many([items[k] for k in ch], device=True)):
+                        if isinstance(r, Exception):
+                            raise r
+                        streams[k] = r
+
+            def dec_all():
+                for ch in dchunks:
+                    for r in j.decode_many([streams[k] for k in ch],
+                                           device_outputs=[(outs[k].data_ptr(), outs[k].numel()) for k in ch]):
+                        if isinstance(r, Exception):
+                            raise r
+
+            _, em = timed(enc_all, 1)
+            _, dm = timed(dec_all, 1)
+            qmpx = sum(p.shape[0] * p.shape[1] for p, _ in host) / 1e6
+            dmpx = sum(host[k][0].shape[0] * host[k][0].shape[1] for k in dec_ids) / 1e6
+            # the batches give each page the bytes and pixels it gets alone
+            sample = dec_ids[:3]
+            same = all(streams[k] == j.encode_u8(host[k][0], host[k][1]) for k in range(min(3, len(host))))
+            same = same and all(np.array_equal(outs[k].cpu().numpy(), j.decode_u8(streams[k])) for k in sample)
+            small = min(range(len(host)), key=lambda k: host[k][0].size)
+            ok = streams[small] == ora.encode(host[small][0], host[small][1])
+            configs["mixed_queue"] = {
+                "pages": len(host), "megapixels": round(qmpx, 1), "batch_pages": B,
+                "gray_pages": sum(1 for p, _ in host if p.ndim == 2),
+                "decoded_pages": len(dec_ids), "encode_s": round(em / 1e3, 3), "decode_s": round(dm / 1e3, 3),
+                "encode_mpx_s": round(qmpx / (em / 1e3), 1), "decode_mpx_s": round(dmpx / (dm / 1e3), 1),
+                "batched_equals_single": bool(same), "parity_vs_oracle_smallest_page": bool(ok)}
+            del devp, outs
+
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib << 20
         a = torch.ones(n, dtype=torch.uint8, device=dev)
@@ -299,6 +401,7 @@ def main():
         "parity_vs_oracle": check,
         "batched_encode": batched,
         "serving": serve,
+        "other_workloads": configs,
     }
     print(json.dumps(line), flush=True)
 

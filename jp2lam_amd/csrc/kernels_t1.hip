@@ -38,6 +38,29 @@ __device__ __forceinline__ void set_priority(int p) {
     else if (p == 1) __builtin_amdgcn_s_setprio(1);
 }
 
+// The decoder's context states live in s[80:98] (context k in s[80 + k])
+// for the whole kernel.  The kernel caps the SGPRs the compiler allocates
+// below 80 (kStateSgprCap), so no compiler code touches the block, and a
+// decision reaches state M0 with s_movrels / s_movreld: no VALU-to-SALU hop
+// and no writelane-to-readlane wait on its critical path.  Listing the
+// block as clobbered here makes the kernel descriptor reserve it.  (An SALU
+// write of M0 needs one wait state before s_movrels.)
+#define kStateSgprCap 80
+__device__ __forceinline__ void states_init(uint32_t cx) {
+    asm volatile(
+        "v_readlane_b32 s80, %0, 0\n\tv_readlane_b32 s81, %0, 1\n\tv_readlane_b32 s82, %0, 2\n\t"
+        "v_readlane_b32 s83, %0, 3\n\tv_readlane_b32 s84, %0, 4\n\tv_readlane_b32 s85, %0, 5\n\t"
+        "v_readlane_b32 s86, %0, 6\n\tv_readlane_b32 s87, %0, 7\n\tv_readlane_b32 s88, %0, 8\n\t"
+        "v_readlane_b32 s89, %0, 9\n\tv_readlane_b32 s90, %0, 10\n\tv_readlane_b32 s91, %0, 11\n\t"
+        "v_readlane_b32 s92, %0, 12\n\tv_readlane_b32 s93, %0, 13\n\tv_readlane_b32 s94, %0, 14\n\t"
+        "v_readlane_b32 s95, %0, 15\n\tv_readlane_b32 s96, %0, 16\n\tv_readlane_b32 s97, %0, 17\n\t"
+        "v_readlane_b32 s98, %0, 18"
+        :
+        : "v"(cx)
+        : "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92", "s93", "s94",
+          "s95", "s96", "s97", "s98");
+}
+
 // Rows 4s-1..4s+4 of a column mask as bits 0..5 (missing rows read 0).
 __device__ __forceinline__ uint32_t win6(uint64_t m, int s) {
     return s ? (uint32_t)(m >> (4 * s - 1)) & 63u : (uint32_t)(m << 1) & 63u;
@@ -54,15 +77,15 @@ __device__ __forceinline__ uint32_t sc_entry(const Tables& t, uint32_t si) { ret
 
 
 // ---- MQ decoder (mq/mod.rs:30-155) -----------------------------------------
-// Coder registers in SGPRs; context lanes hold the state's table entry with
-// the MPS in bit 31 (as in the encoder).  The segment is read through a
+// Coder registers in SGPRs; context k's state is its table entry with the
+// MPS in bit 31 (as in the encoder), kept in s[80 + k].  The segment is read through a
 // 128-byte window of two VGPRs (lane i of w0/w1 = byte dbase + i / + 64 + i,
 // 0xFF past the end, which the decoder reads as the reference does,
 // mq/mod.rs:152-154); the scan slides it before each column, which reads
 // fewer than 24 bytes, so a decision never leaves it.
 struct MqDec {
     uint32_t a, c, ct, bp, cur;  // cur = byte at bp
-    uint32_t cx, tab;            // VGPRs
+    uint32_t tab;                // VGPR: transition table
     uint32_t w0, w1, dbase;
     const uint8_t* data;
     uint32_t len;
@@ -118,10 +141,10 @@ __device__ __forceinline__ void dec_start(MqDec& d) {
 __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t ctx) {
     uint32_t bit, e, q, t, k, n0, n1;
     asm volatile(
-        "v_readlane_b32 %[e], %[cx], m0\n\t"
+        "s_lshr_b32 %[t], %[c], 16\n\t"  // also the M0 wait state
+        "s_movrels_b32 %[e], s80\n\t"
         "s_and_b32 %[q], %[e], 0xffff\n\t"
         "s_sub_u32 %[a], %[a], %[q]\n\t"
-        "s_lshr_b32 %[t], %[c], 16\n\t"
         "s_cmp_lt_u32 %[t], %[q]\n\t"
         "s_cbranch_scc1 .Llower%=\n\t"
         "s_lshl_b32 %[t], %[q], 16\n\t"
@@ -152,7 +175,7 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_and_b32 %[t], %[t], 63\n\t"
         "v_readlane_b32 %[t], %[tab], %[t]\n\t"
         "s_or_b32 %[t], %[t], %[n0]\n\t"
-        "v_writelane_b32 %[cx], %[t], m0\n\t"
+        "s_movreld_b32 s80, %[t]\n\t"
         "s_flbit_i32_b32 %[t], %[a]\n\t"
         "s_sub_u32 %[t], %[t], 16\n\t"
         "s_cmp_gt_u32 %[t], %[ct]\n\t"
@@ -202,8 +225,7 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_mov_b32 %[ct], 8\n\t"
         "s_branch .Lrn2%=\n"
         ".Ldone%=:"
-        : [a] "+s"(d.a), [c] "+s"(d.c), [ct] "+s"(d.ct), [bp] "+s"(d.bp), [cur] "+s"(d.cur), [cx] "+v"(d.cx),
-          [m0] "+{m0}"(ctx), [bit] "=&s"(bit), [e] "=&s"(e), [q] "=&s"(q), [t] "=&s"(t), [k] "=&s"(k),
+        : [a] "+s"(d.a), [c] "+s"(d.c), [ct] "+s"(d.ct), [bp] "+s"(d.bp), [cur] "+s"(d.cur), [m0] "+{m0}"(ctx), [bit] "=&s"(bit), [e] "=&s"(e), [q] "=&s"(q), [t] "=&s"(t), [k] "=&s"(k),
           [n0] "=&s"(n0), [n1] "=&s"(n1)
         : [tab] "v"(d.tab), [w0] "v"(d.w0), [w1] "v"(d.w1), [db] "s"(d.dbase)
         : "scc");
@@ -694,7 +716,9 @@ __device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t 
 // ---- MQ encoder (mq/mod.rs:163-432) --------------------------------------------
 // Coder registers in SGPRs.  cx lane k holds context k's current state as
 // its table entry (qe | nmps << 16 | nlps << 22 | switch << 28) with the
-// MPS in bit 31, so a decision needs one readlane.  Output bytes collect in
+// MPS in bit 31, so a decision needs one readlane.  (Keeping the states in
+// an SGPR block as the decoder does measured no faster here: the state
+// write sits early enough in the renormalising path to hide its latency.)  Output bytes collect in
 // a 128-byte window of two VGPRs (lane i of w0/w1 = byte wbase + i / + 64 + i);
 // the lower 64 are stored with one coalesced write once final.
 struct MqE {
@@ -1290,7 +1314,7 @@ __global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __res
 // passes and writes the reconstructed coefficients into the plane:
 // dequantised f32 (sign * (|q| + 0.5)) * step for 9/7 (scalar.rs:233-240),
 // i32 for 5/3.  Blocks that are not included are written as zeros.
-__global__ __launch_bounds__(64) void t1_decode_kernel(const DecodeJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap))) void t1_decode_kernel(const DecodeJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                        const uint8_t* __restrict__ payload) {
     const int lane = threadIdx.x;
     const DecodeJob J = jobs[blockIdx.x];
@@ -1309,7 +1333,7 @@ __global__ __launch_bounds__(64) void t1_decode_kernel(const DecodeJob* __restri
         cd.m.len = J.data_len;
         cd.m.tab = t.mq;
         // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
-        cd.m.cx = (uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64);
+        states_init((uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64));
         dec_start(cd.m);
         Col st = {0, 0, 0, 0};
         const int mbp = (int)J.mb - (int)J.zbp;

@@ -71,34 +71,54 @@ def document_page(width: int = 2550, height: int = 3300, seed: int = 2) -> np.nd
     return np.ascontiguousarray(img)
 
 
-def _pink_noise(rng: np.random.Generator, h: int, w: int, sigma: float) -> np.ndarray:
-    white = rng.normal(0.0, 1.0, size=(h, w))
-    f = np.fft.rfft2(white)
-    fy = np.fft.fftfreq(h)[:, None]
-    fx = np.fft.rfftfreq(w)[None, :]
+def _pink_noise(rng: np.random.Generator, n: int, h: int, w: int, sigma: float) -> np.ndarray:
+    """n independent (h, w) fields of 1/f noise with standard deviation sigma."""
+    try:  # multithreaded FFTs when scipy is present; numpy's otherwise
+        from scipy import fft as _fft
+
+        kw = {"workers": -1}
+    except ImportError:  # pragma: no cover
+        _fft = np.fft
+        kw = {}
+    white = rng.standard_normal(size=(n, h, w), dtype=np.float32)
+    f = _fft.rfft2(white, axes=(1, 2), **kw)
+    fy = np.fft.fftfreq(h).astype(np.float32)[:, None]
+    fx = np.fft.rfftfreq(w).astype(np.float32)[None, :]
     r = np.sqrt(fx * fx + fy * fy)
     r[0, 0] = 1.0
-    f = f / r
-    f[0, 0] = 0.0
-    n = np.fft.irfft2(f, s=(h, w))
-    std = n.std()
-    return n * (sigma / std) if std > 0 else n
+    f *= (1.0 / r)[None]
+    f[:, 0, 0] = 0.0
+    out = _fft.irfft2(f, s=(h, w), axes=(1, 2), **kw)
+    for c in range(n):
+        std = float(out[c].std())
+        if std > 0:
+            out[c] *= np.float32(sigma / std)
+    return out
 
 
 def photo_page(width: int = 4000, height: int = 3000, seed: int = 3) -> np.ndarray:
     rng = np.random.default_rng(seed)
-    yy, xx = np.mgrid[0:height, 0:width].astype(np.float32)
+    x = np.arange(width, dtype=np.float64)
+    y = np.arange(height, dtype=np.float64)
+    noise = _pink_noise(rng, 3, height, width, 12.0)
     out = np.empty((height, width, 3), dtype=np.uint8)
     for c in range(3):
-        acc = np.zeros((height, width), dtype=np.float32)
-        for _ in range(8):
+        # sum of 8 oriented sinusoids amp * sin(k (x cos t + y sin t) + phase),
+        # split by angle addition into one (H x 16) @ (16 x W) product
+        rows = np.empty((height, 16), dtype=np.float64)
+        cols = np.empty((16, width), dtype=np.float64)
+        for i in range(8):
             period = float(rng.uniform(16.0, 800.0))
             amp = float(rng.uniform(10.0, 40.0))
             theta = float(rng.uniform(0.0, np.pi))
             phase = float(rng.uniform(0.0, 2.0 * np.pi))
-            k = np.float32(2.0 * np.pi / period)
-            acc += np.float32(amp) * np.sin(k * (xx * np.float32(np.cos(theta)) + yy * np.float32(np.sin(theta))) + np.float32(phase))
-        acc += _pink_noise(rng, height, width, 12.0).astype(np.float32)
+            k = 2.0 * np.pi / period
+            ax = k * np.cos(theta) * x
+            by = k * np.sin(theta) * y + phase
+            rows[:, 2 * i], rows[:, 2 * i + 1] = np.cos(by), np.sin(by)
+            cols[2 * i], cols[2 * i + 1] = amp * np.sin(ax), amp * np.cos(ax)
+        acc = (rows @ cols).astype(np.float32)
+        acc += noise[c]
         acc += np.float32(rng.uniform(60.0, 190.0))
         out[:, :, c] = _clamp_u8(acc)
     return np.ascontiguousarray(out)
