@@ -884,7 +884,7 @@ void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
                     pos += blocks[i].len;
                 }
             }
-    if (pos != plen) fail("tile payload has trailing bytes after packet parse");
+    if (pos != plen) fail("tile payload has " + std::to_string(plen - pos) + " trailing bytes after packet parse");
 }
 
 }  // namespace j2l

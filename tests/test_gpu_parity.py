@@ -201,6 +201,15 @@ def expect_same_decode(ora, data):
     return got
 
 
+def test_decode_header_ending_in_ff_fails_like_the_reference(gpu, ora):
+    # decode/t2.rs:450-452 counts no stuffing byte after a header ending in
+    # 0xFF; this stream (block 1: 37 passes, 511 bytes) hits it
+    px = synth.photo_page(40, 33, seed=5)
+    data = j.encode_u8(px, 99)
+    assert data == ora.encode(px, 99)
+    assert expect_same_decode(ora, data) is None
+
+
 @pytest.mark.parametrize("name,make,qs", CASES, ids=[c[0] for c in CASES])
 def test_decode_bit_exact(gpu, ora, name, make, qs):
     px = make()

@@ -246,11 +246,31 @@ def test_openjpeg_decodes_oracle_streams(ora, q):
 
 def test_rgb_lossless_openjpeg(ora):
     PIL = pytest.importorskip("PIL.Image")
-    px = synth.photo_page(40, 33, seed=5)
+    px = synth.photo_page(40, 33, seed=6)
     data = ora.encode(px, 99)
     theirs = np.array(PIL.open(io.BytesIO(data))).astype(int)
     ours = ora.decode(data).astype(int)
     assert np.abs(theirs - ours).max() <= 1
+
+
+def test_header_ending_in_ff_misparsed_like_the_reference(ora):
+    # A packet header whose last byte is 0xFF is followed by the stuffed byte
+    # the encoder's flush writes (native/t2.rs:60-66, as OpenJPEG does); the
+    # reference decoder counts only the bytes it read (decode/t2.rs:450-452),
+    # so it starts the body one byte early and fails at the end of the
+    # payload.  Here block 1 (37 passes, a 511-byte segment) ends its packet
+    # header in 0xFF.  OpenJPEG decodes the same stream.
+    from oracle_ctypes import OracleError
+
+    PIL = pytest.importorskip("PIL.Image")
+    px = synth.photo_page(40, 33, seed=5)
+    data, enc = ora.encode(px, 99, keep=True)
+    assert enc.info(1)[10] == 37 and enc.info(1)[13] == 511
+    with pytest.raises(OracleError) as e:
+        ora.decode(data)
+    assert e.value.code == 3 and "trailing bytes after packet parse" in e.value.msg
+    theirs = np.array(PIL.open(io.BytesIO(data))).astype(int)
+    assert np.abs(theirs - px.astype(int)).max() <= 2
 
 
 OPJ = shutil.which("opj_compress") or "/opt/conda/bin/opj_compress"
