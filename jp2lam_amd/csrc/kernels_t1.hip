@@ -82,11 +82,12 @@ __device__ __forceinline__ uint32_t sc_entry(const Tables& t, uint32_t si) { ret
 // 128-byte window of two VGPRs (lane i of w0/w1 = byte dbase + i / + 64 + i,
 // 0xFF past the end, which the decoder reads as the reference does,
 // mq/mod.rs:152-154); the scan slides it before each column, which reads
-// fewer than 24 bytes, so a decision never leaves it.
+// fewer than 24 bytes, so a decision never leaves it.  The next 64 bytes
+// (w2) are loaded one slide ahead, so a slide never waits on memory.
 struct MqDec {
     uint32_t a, c, ct, bp, cur;  // cur = byte at bp
     uint32_t tab;                // VGPR: transition table
-    uint32_t w0, w1, dbase;
+    uint32_t w0, w1, w2, dbase;
     const uint8_t* data;
     uint32_t len;
 };
@@ -98,12 +99,14 @@ __device__ __forceinline__ void dec_window_init(MqDec& d) {
     d.dbase = 0;
     d.w0 = dec_load64(d, 0);
     d.w1 = dec_load64(d, 64);
+    d.w2 = dec_load64(d, 128);
 }
 __device__ __forceinline__ void dec_slide(MqDec& d) {
     if (d.bp + 24u >= d.dbase + 128u) {
         d.w0 = d.w1;
+        d.w1 = d.w2;
         d.dbase += 64u;
-        d.w1 = dec_load64(d, d.dbase + 64u);
+        d.w2 = dec_load64(d, d.dbase + 128u);
     }
 }
 // Start of decoding (mq/mod.rs:46-60): c from the first byte, one byte-in,
@@ -162,17 +165,15 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_cselect_b32 %[k], 0, -1\n\t"
         "s_mov_b32 %[a], %[q]\n"
         ".Lcommon%=:\n\t"                    // k = all ones on LPS
-        "s_lshr_b32 %[bit], %[e], 31\n\t"
-        "s_and_b32 %[t], %[k], 1\n\t"
-        "s_xor_b32 %[bit], %[bit], %[t]\n\t"
+        "s_xor_b32 %[bit], %[e], %[k]\n\t"   // bit 31: MPS ^ LPS = the decision
+        "s_lshr_b32 %[bit], %[bit], 31\n\t"
         "s_lshl_b32 %[n0], %[e], 3\n\t"
         "s_and_b32 %[n0], %[n0], %[k]\n\t"
         "s_xor_b32 %[n0], %[n0], %[e]\n\t"
         "s_and_b32 %[n0], %[n0], 0x80000000\n\t"
         "s_cmp_lg_u32 %[k], 0\n\t"
-        "s_cselect_b32 %[t], 22, 16\n\t"
-        "s_lshr_b32 %[t], %[e], %[t]\n\t"
-        "s_and_b32 %[t], %[t], 63\n\t"
+        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
+        "s_bfe_u32 %[t], %[e], %[t]\n\t"
         "v_readlane_b32 %[t], %[tab], %[t]\n\t"
         "s_or_b32 %[t], %[t], %[n0]\n\t"
         "s_movreld_b32 s80, %[t]\n\t"
@@ -227,7 +228,7 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         ".Ldone%=:"
         : [a] "+s"(d.a), [c] "+s"(d.c), [ct] "+s"(d.ct), [bp] "+s"(d.bp), [cur] "+s"(d.cur), [m0] "+{m0}"(ctx), [bit] "=&s"(bit), [e] "=&s"(e), [q] "=&s"(q), [t] "=&s"(t), [k] "=&s"(k),
           [n0] "=&s"(n0), [n1] "=&s"(n1)
-        : [tab] "v"(d.tab), [w0] "v"(d.w0), [w1] "v"(d.w1), [db] "s"(d.dbase)
+        : [tab] "v"(d.tab), [w0] "v"(d.w0), [w1] "v"(d.w1), [db] "s"(d.dbase), [f16] "s"(0x60010u)
         : "scc");
     return bit;
 }
@@ -277,7 +278,7 @@ __device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
     uint32_t newsig = 0;
     const int nstripes = (h + 3) >> 2;
     for (int s = 0; s < nstripes; s++) {
-        const int nrows = min(4, h - 4 * s);
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = (1u << nrows) - 1u;
         uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
         uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
@@ -300,7 +301,9 @@ __device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
             uint32_t sigW = (Pw & 63u) | (newW << 1), sgnW = ((Pw >> 6) & 63u) | (newWs << 1);
             uint32_t sigE = Pe & 63u, sgnE = (Pe >> 6) & 63u;
             uint32_t newC = 0, newCs = 0, visC = 0;
-            for (int ci = 0; ci < nrows; ci++) {
+#pragma unroll
+            for (int ci = 0; ci < 4; ci++) {
+                if (ci >= nrows) break;
                 const uint32_t r = (uint32_t)ci + 1;
                 if ((sigC >> r) & 1u) continue;
                 uint32_t effC = sigC | (newC << 1);
@@ -335,27 +338,31 @@ __device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
 
 // Magnitude refinement pass (t1.rs:914-965, decode/t1.rs:268-315): members
 // and contexts are independent of the pass's own decisions, so they are
-// formed lane-parallel; only the coding is serial.
+// formed lane-parallel; only the coding is serial.  Per stripe-column the
+// word holds the member rows (bits 0..3; rows past the block are never
+// members) and each row's context index (5 bits at 4 + 5 ci), so the row
+// loop unrolls and a decision's context is one bit-field extract.
 template <bool DEC>
 __device__ void mr_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t& plane, int w, int h) {
     const int lane = threadIdx.x;
     const int nstripes = (h + 3) >> 2;
     for (int s = 0; s < nstripes; s++) {
-        const int nrows = min(4, h - 4 * s);
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = (1u << nrows) - 1u;
         uint32_t sig6 = win6(st.sig, s);
         uint32_t sig4 = (sig6 >> 1) & 15u, vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
-        uint32_t ref4 = (uint32_t)(st.ref >> (4 * s)) & 15u, b4 = (uint32_t)(bits >> (4 * s)) & 15u;
+        uint32_t ref4 = (uint32_t)(st.ref >> (4 * s)) & 15u;
         uint32_t mem = sig4 & ~vis4 & rmask;
         uint32_t w6 = __shfl_up(sig6, 1, 64), e6 = __shfl_down(sig6, 1, 64);
         if (lane == 0) w6 = 0;
         if (lane == 63) e6 = 0;
-        uint32_t Q = mem | (b4 << 12);
+        uint32_t Q = mem;
+#pragma unroll
         for (int ci = 0; ci < 4; ci++) {
-            uint32_t r = (uint32_t)ci + 1;
-            uint32_t any = nbmask(w6, sig6, e6, r) != 0;
-            uint32_t off = ((ref4 >> ci) & 1u) ? 2u : any;
-            Q |= off << (4 + 2 * ci);
+            const uint32_t r = (uint32_t)ci + 1;
+            const uint32_t any = nbmask(w6, sig6, e6, r) != 0;
+            const uint32_t ctx = ((ref4 >> ci) & 1u) ? CTX_MAG + 2u : CTX_MAG + any;
+            Q |= ctx << (4 + 5 * ci);
         }
         uint64_t M = ballot(lane < w && mem != 0);
         uint32_t upd = 0;
@@ -363,16 +370,16 @@ __device__ void mr_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits,
             int x = __builtin_ctzll(M);
             M &= M - 1;
             cd.column();
-            uint32_t q = rl(Q, x), ones = 0;
-            for (int ci = 0; ci < nrows; ci++) {
-                if (!((q >> ci) & 1u)) continue;
-                uint32_t bit = cd.code(t, CTX_MAG + ((q >> (4 + 2 * ci)) & 3u), (q >> (12 + ci)) & 1u);
-                ones |= bit << ci;
-            }
-            if (DEC) upd = wl(upd, ones, x);
+            const uint32_t q = rl(Q, x);
+            uint32_t ones = 0;
+#pragma unroll
+            for (int ci = 0; ci < 4; ci++)
+                if ((q >> ci) & 1u) ones |= cd.code(t, (q >> (4 + 5 * ci)) & 31u, 0) << ci;
+            upd = wl(upd, ones, x);
         }
-        if (DEC) plane |= (uint64_t)(upd & 15u) << (4 * s);
+        plane |= (uint64_t)(upd & 15u) << (4 * s);
     }
+    (void)bits;
     st.ref |= st.sig & ~st.vis;
 }
 
@@ -385,7 +392,7 @@ __device__ uint32_t cl_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
     uint32_t newsig = 0;
     const int nstripes = (h + 3) >> 2;
     for (int s = 0; s < nstripes; s++) {
-        const int nrows = min(4, h - 4 * s);
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = (1u << nrows) - 1u;
         uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
         uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
@@ -448,7 +455,10 @@ __device__ uint32_t cl_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
                     ci0 = (int)rlen + 1;
                 }
             }
-            for (int ci = ci0; ci < nrows; ci++) {
+#pragma unroll
+            for (int ci = 0; ci < 4; ci++) {
+                if (ci >= nrows) break;
+                if (ci < ci0) continue;
                 const uint32_t r = (uint32_t)ci + 1;
                 if (((sigC >> r) & 1u) || ((visC >> ci) & 1u)) continue;
                 uint32_t effC = sigC | (newC << 1);
@@ -572,7 +582,7 @@ __device__ __forceinline__ uint32_t form_spp(const Tables& t, Col& st, uint64_t 
     const int nstripes = (h + 3) >> 2;
     uint32_t total = 0, ns = 0;
     for (int s = 0; s < nstripes; s++) {
-        const int nrows = min(4, h - 4 * s);
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = valid ? (1u << nrows) - 1u : 0u;
         const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
         const uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
@@ -631,7 +641,7 @@ __device__ __forceinline__ uint32_t form_mrp(Col& st, uint64_t bits, int w, int 
     const int nstripes = (h + 3) >> 2;
     uint32_t total = 0;
     for (int s = 0; s < nstripes; s++) {
-        const int nrows = min(4, h - 4 * s);
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = valid ? (1u << nrows) - 1u : 0u;
         const uint32_t sig6 = win6(st.sig, s);
         const uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u, ref4 = (uint32_t)(st.ref >> (4 * s)) & 15u;
@@ -664,7 +674,7 @@ __device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t 
     const int nstripes = (h + 3) >> 2;
     uint32_t total = 0, ns = 0;
     for (int s = 0; s < nstripes; s++) {
-        const int nrows = min(4, h - 4 * s);
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = valid ? (1u << nrows) - 1u : 0u;
         const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
         const uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
@@ -752,21 +762,20 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         // exchange to_qe = (a < qe) XOR lps; MPS: a = qe if a < qe else
         // c += qe; LPS: c += qe if a < qe else a = qe; state NMPS / NLPS,
         // MPS ^= SWITCH on LPS
-        "s_cmp_lt_u32 %[a], %[q]\n\t"
-        "s_cselect_b32 %[t], -1, 0\n\t"
+        // (a, qe < 2^16, so bit 31 of a - qe is a < qe)
+        "s_sub_u32 %[t], %[a], %[q]\n\t"
         "s_xor_b32 %[t], %[t], %[u]\n\t"
         "s_cmp_lt_i32 %[t], 0\n\t"
         "s_cselect_b32 %[a], %[q], %[a]\n\t"
         "s_cselect_b32 %[t], 0, %[q]\n\t"
         "s_add_u32 %[c], %[c], %[t]\n\t"
-        "s_lshl_b32 %[q], %[e], 3\n\t"
         "s_cmp_lt_i32 %[u], 0\n\t"
-        "s_cselect_b32 %[t], 22, 16\n\t"
-        "s_cselect_b32 %[q], %[q], 0\n\t"
-        "s_xor_b32 %[q], %[q], %[e]\n\t"
-        "s_lshr_b32 %[t], %[e], %[t]\n\t"
-        "s_and_b32 %[t], %[t], 63\n\t"
+        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
+        "s_bfe_u32 %[t], %[e], %[t]\n\t"
         "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "s_lshl_b32 %[q], %[e], 3\n\t"               // bit 31: SWITCH
+        "s_and_b32 %[q], %[q], %[u]\n\t"             // SWITCH on LPS only
+        "s_xor_b32 %[q], %[q], %[e]\n\t"             // bit 31: the new MPS
         "s_and_b32 %[q], %[q], 0x80000000\n\t"
         "s_or_b32 %[t], %[t], %[q]\n\t"
         "v_writelane_b32 %[cx], %[t], m0\n\t"
@@ -820,7 +829,7 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
           [cx] "+v"(m.cx), [w0] "+v"(m.w0), [w1] "+v"(m.w1), [m0] "+{m0}"(ctx), [e] "=&s"(e), [q] "=&s"(q),
           [u] "=&s"(u), [t] "=&s"(t)
-        : [d] "s"(d31), [tab] "v"(m.tab), [wb] "s"(m.wbase)
+        : [d] "s"(d31), [tab] "v"(m.tab), [wb] "s"(m.wbase), [f16] "s"(0x60010u)
         : "scc");
 }
 // Store the window's lower 64 bytes once they are all committed.
@@ -1318,7 +1327,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
                                                        const uint8_t* __restrict__ payload) {
     const int lane = threadIdx.x;
     const DecodeJob J = jobs[blockIdx.x];
-    const int w = J.w, h = J.h;
+    const int w = __builtin_amdgcn_readfirstlane(J.w), h = __builtin_amdgcn_readfirstlane(J.h);
     const bool lossy = J.step > 0.0f;
     uint32_t mag[64];
 #pragma unroll
