@@ -153,38 +153,14 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_lshl_b32 %[t], %[q], 16\n\t"
         "s_sub_u32 %[c], %[c], %[t]\n\t"
         "s_bitcmp1_b32 %[a], 15\n\t"
-        "s_cbranch_scc0 .Lup%=\n\t"
-        "s_lshr_b32 %[bit], %[e], 31\n\t"
+        "s_cbranch_scc0 .Lup%=\n\t"         // renormalising upper path: the one taken branch
+        "s_lshr_b32 %[bit], %[e], 31\n\t"   // MPS, a stays >= 0x8000
         "s_branch .Ldone%=\n"
-        ".Lup%=:\n\t"                        // upper sub-interval, a < 0x8000: LPS iff a < qe
-        "s_cmp_lt_u32 %[a], %[q]\n\t"
-        "s_cselect_b32 %[k], -1, 0\n\t"
-        "s_branch .Lcommon%=\n"
         ".Llower%=:\n\t"                     // lower sub-interval: a = qe; LPS iff a >= qe
         "s_cmp_lt_u32 %[a], %[q]\n\t"
         "s_cselect_b32 %[k], 0, -1\n\t"
-        "s_mov_b32 %[a], %[q]\n"
-        ".Lcommon%=:\n\t"                    // k = all ones on LPS
-        "s_xor_b32 %[bit], %[e], %[k]\n\t"   // bit 31: MPS ^ LPS = the decision
-        "s_lshr_b32 %[bit], %[bit], 31\n\t"
-        "s_lshl_b32 %[n0], %[e], 3\n\t"
-        "s_and_b32 %[n0], %[n0], %[k]\n\t"
-        "s_xor_b32 %[n0], %[n0], %[e]\n\t"
-        "s_and_b32 %[n0], %[n0], 0x80000000\n\t"
-        "s_cmp_lg_u32 %[k], 0\n\t"
-        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
-        "s_bfe_u32 %[t], %[e], %[t]\n\t"
-        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
-        "s_or_b32 %[t], %[t], %[n0]\n\t"
-        "s_movreld_b32 s80, %[t]\n\t"
-        "s_flbit_i32_b32 %[t], %[a]\n\t"
-        "s_sub_u32 %[t], %[t], 16\n\t"
-        "s_cmp_gt_u32 %[t], %[ct]\n\t"
-        "s_cbranch_scc1 .Lrn%=\n\t"
-        "s_lshl_b32 %[a], %[a], %[t]\n\t"
-        "s_lshl_b32 %[c], %[c], %[t]\n\t"
-        "s_sub_u32 %[ct], %[ct], %[t]\n\t"
-        "s_branch .Ldone%=\n"
+        "s_mov_b32 %[a], %[q]\n\t"
+        "s_branch .Lcommon%=\n"
         ".Lrn%=:\n\t"                        // shift with byte-in whenever ct is 0
         "s_cmp_eq_u32 %[ct], 0\n\t"
         "s_cbranch_scc1 .Lbytein%=\n"
@@ -225,6 +201,29 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_add_u32 %[c], %[c], %[n0]\n\t"
         "s_mov_b32 %[ct], 8\n\t"
         "s_branch .Lrn2%=\n"
+        ".Lup%=:\n\t"                        // upper sub-interval, a < 0x8000: LPS iff a < qe
+        "s_cmp_lt_u32 %[a], %[q]\n\t"
+        "s_cselect_b32 %[k], -1, 0\n"
+        ".Lcommon%=:\n\t"                    // k = all ones on LPS
+        "s_xor_b32 %[bit], %[e], %[k]\n\t"   // bit 31: MPS ^ LPS = the decision
+        "s_lshr_b32 %[bit], %[bit], 31\n\t"
+        "s_lshl_b32 %[n0], %[e], 3\n\t"
+        "s_and_b32 %[n0], %[n0], %[k]\n\t"
+        "s_xor_b32 %[n0], %[n0], %[e]\n\t"
+        "s_and_b32 %[n0], %[n0], 0x80000000\n\t"
+        "s_cmp_lg_u32 %[k], 0\n\t"
+        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
+        "s_bfe_u32 %[t], %[e], %[t]\n\t"
+        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "s_or_b32 %[t], %[t], %[n0]\n\t"
+        "s_movreld_b32 s80, %[t]\n\t"
+        "s_flbit_i32_b32 %[t], %[a]\n\t"
+        "s_sub_u32 %[t], %[t], 16\n\t"
+        "s_cmp_gt_u32 %[t], %[ct]\n\t"
+        "s_cbranch_scc1 .Lrn%=\n\t"
+        "s_lshl_b32 %[a], %[a], %[t]\n\t"
+        "s_lshl_b32 %[c], %[c], %[t]\n\t"
+        "s_sub_u32 %[ct], %[ct], %[t]\n"
         ".Ldone%=:"
         : [a] "+s"(d.a), [c] "+s"(d.c), [ct] "+s"(d.ct), [bp] "+s"(d.bp), [cur] "+s"(d.cur), [m0] "+{m0}"(ctx), [bit] "=&s"(bit), [e] "=&s"(e), [q] "=&s"(q), [t] "=&s"(t), [k] "=&s"(k),
           [n0] "=&s"(n0), [n1] "=&s"(n1)
@@ -754,37 +753,10 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_lshl_b32 %[t], %[a], 16\n\t"
         "s_andn2_b32 %[t], %[t], %[u]\n\t"
         "s_cmp_lt_i32 %[t], 0\n\t"
-        "s_cbranch_scc0 .Lslow%=\n\t"
+        "s_cbranch_scc0 .Lslow%=\n\t"       // renormalising path: the one taken branch
         "s_add_u32 %[c], %[c], %[q]\n\t"
         "s_branch .Ldone%=\n"
-        ".Lslow%=:\n\t"
-        // renormalising decision, MPS or LPS, branch-free: conditional
-        // exchange to_qe = (a < qe) XOR lps; MPS: a = qe if a < qe else
-        // c += qe; LPS: c += qe if a < qe else a = qe; state NMPS / NLPS,
-        // MPS ^= SWITCH on LPS
-        // (a, qe < 2^16, so bit 31 of a - qe is a < qe)
-        "s_sub_u32 %[t], %[a], %[q]\n\t"
-        "s_xor_b32 %[t], %[t], %[u]\n\t"
-        "s_cmp_lt_i32 %[t], 0\n\t"
-        "s_cselect_b32 %[a], %[q], %[a]\n\t"
-        "s_cselect_b32 %[t], 0, %[q]\n\t"
-        "s_add_u32 %[c], %[c], %[t]\n\t"
-        "s_cmp_lt_i32 %[u], 0\n\t"
-        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
-        "s_bfe_u32 %[t], %[e], %[t]\n\t"
-        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
-        "s_lshl_b32 %[q], %[e], 3\n\t"               // bit 31: SWITCH
-        "s_and_b32 %[q], %[q], %[u]\n\t"             // SWITCH on LPS only
-        "s_xor_b32 %[q], %[q], %[e]\n\t"             // bit 31: the new MPS
-        "s_and_b32 %[q], %[q], 0x80000000\n\t"
-        "s_or_b32 %[t], %[t], %[q]\n\t"
-        "v_writelane_b32 %[cx], %[t], m0\n\t"
-        "s_flbit_i32_b32 %[t], %[a]\n\t"
-        "s_sub_u32 %[t], %[t], 16\n"
-        // renormalise by t bits, a byte out whenever ct reaches zero
-        ".Lrn%=:\n\t"
-        "s_cmp_lt_u32 %[t], %[ct]\n\t"
-        "s_cbranch_scc1 .Lrnlast%=\n\t"
+        ".Lbyte%=:\n\t"                      // t >= ct: shift by ct and move a byte out
         "s_lshl_b32 %[a], %[a], %[ct]\n\t"
         "s_lshl_b32 %[c], %[c], %[ct]\n\t"
         "s_sub_u32 %[t], %[t], %[ct]\n\t"
@@ -821,7 +793,34 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_add_u32 %[pos], %[pos], 1\n\t"
         "s_and_b32 %[last], %[q], 0xff\n\t"
         "s_branch .Lrn%=\n"
-        ".Lrnlast%=:\n\t"
+        ".Lslow%=:\n\t"
+        // renormalising decision, MPS or LPS, branch-free: conditional
+        // exchange to_qe = (a < qe) XOR lps; MPS: a = qe if a < qe else
+        // c += qe; LPS: c += qe if a < qe else a = qe; state NMPS / NLPS,
+        // MPS ^= SWITCH on LPS
+        // (a, qe < 2^16, so bit 31 of a - qe is a < qe)
+        "s_sub_u32 %[t], %[a], %[q]\n\t"
+        "s_xor_b32 %[t], %[t], %[u]\n\t"
+        "s_cmp_lt_i32 %[t], 0\n\t"
+        "s_cselect_b32 %[a], %[q], %[a]\n\t"
+        "s_cselect_b32 %[t], 0, %[q]\n\t"
+        "s_add_u32 %[c], %[c], %[t]\n\t"
+        "s_cmp_lt_i32 %[u], 0\n\t"
+        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
+        "s_bfe_u32 %[t], %[e], %[t]\n\t"
+        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "s_lshl_b32 %[q], %[e], 3\n\t"               // bit 31: SWITCH
+        "s_and_b32 %[q], %[q], %[u]\n\t"             // SWITCH on LPS only
+        "s_xor_b32 %[q], %[q], %[e]\n\t"             // bit 31: the new MPS
+        "s_and_b32 %[q], %[q], 0x80000000\n\t"
+        "s_or_b32 %[t], %[t], %[q]\n\t"
+        "v_writelane_b32 %[cx], %[t], m0\n\t"
+        "s_flbit_i32_b32 %[t], %[a]\n\t"
+        "s_sub_u32 %[t], %[t], 16\n"
+        // renormalise by t bits, a byte out whenever ct reaches zero
+        ".Lrn%=:\n\t"
+        "s_cmp_lt_u32 %[t], %[ct]\n\t"
+        "s_cbranch_scc0 .Lbyte%=\n\t"
         "s_lshl_b32 %[a], %[a], %[t]\n\t"
         "s_lshl_b32 %[c], %[c], %[t]\n\t"
         "s_sub_u32 %[ct], %[ct], %[t]\n"
