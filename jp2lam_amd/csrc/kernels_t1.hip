@@ -23,12 +23,15 @@ namespace {
 
 constexpr int CTX_ZC = 0, CTX_MAG = 14, CTX_AGG = 17, CTX_UNI = 18;
 
+// Lane index within the wavefront (the encoder runs two waves per block).
+__device__ __forceinline__ uint32_t lid() { return threadIdx.x & 63u; }
+
 __device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 // Write a wave-uniform value into one lane of a VGPR (v_cmp + v_cndmask).
 __device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t val, uint32_t lane) {
-    return threadIdx.x == lane ? val : v;
+    return lid() == lane ? val : v;
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 // Wave issue priority within its SIMD (s_setprio takes an immediate).
@@ -92,7 +95,7 @@ struct MqDec {
     uint32_t len;
 };
 __device__ __forceinline__ uint32_t dec_load64(const MqDec& d, uint32_t base) {
-    const uint32_t i = base + threadIdx.x;
+    const uint32_t i = base + lid();
     return i < d.len ? (uint32_t)d.data[i] : 0xffu;
 }
 __device__ __forceinline__ void dec_window_init(MqDec& d) {
@@ -149,31 +152,31 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_and_b32 %[q], %[e], 0xffff\n\t"
         "s_sub_u32 %[a], %[a], %[q]\n\t"
         "s_cmp_lt_u32 %[t], %[q]\n\t"
-        "s_cbranch_scc1 .Llower%=\n\t"
+        "s_cbranch_scc1 1f\n\t"
         "s_lshl_b32 %[t], %[q], 16\n\t"
         "s_sub_u32 %[c], %[c], %[t]\n\t"
         "s_bitcmp1_b32 %[a], 15\n\t"
-        "s_cbranch_scc0 .Lup%=\n\t"         // renormalising upper path: the one taken branch
+        "s_cbranch_scc0 7f\n\t"         // renormalising upper path: the one taken branch
         "s_lshr_b32 %[bit], %[e], 31\n\t"   // MPS, a stays >= 0x8000
-        "s_branch .Ldone%=\n"
-        ".Llower%=:\n\t"                     // lower sub-interval: a = qe; LPS iff a >= qe
+        "s_branch 9f\n"
+        "1:\n\t"                     // lower sub-interval: a = qe; LPS iff a >= qe
         "s_cmp_lt_u32 %[a], %[q]\n\t"
         "s_cselect_b32 %[k], 0, -1\n\t"
         "s_mov_b32 %[a], %[q]\n\t"
-        "s_branch .Lcommon%=\n"
-        ".Lrn%=:\n\t"                        // shift with byte-in whenever ct is 0
+        "s_branch 8f\n"
+        "2:\n\t"                        // shift with byte-in whenever ct is 0
         "s_cmp_eq_u32 %[ct], 0\n\t"
-        "s_cbranch_scc1 .Lbytein%=\n"
-        ".Lrn2%=:\n\t"
+        "s_cbranch_scc1 4f\n"
+        "3:\n\t"
         "s_min_u32 %[k], %[t], %[ct]\n\t"
         "s_lshl_b32 %[a], %[a], %[k]\n\t"
         "s_lshl_b32 %[c], %[c], %[k]\n\t"
         "s_sub_u32 %[ct], %[ct], %[k]\n\t"
         "s_sub_u32 %[t], %[t], %[k]\n\t"
         "s_cmp_eq_u32 %[t], 0\n\t"
-        "s_cbranch_scc0 .Lrn%=\n\t"
-        "s_branch .Ldone%=\n"
-        ".Lbytein%=:\n\t"                    // mq/mod.rs:133-155
+        "s_cbranch_scc0 2b\n\t"
+        "s_branch 9f\n"
+        "4:\n\t"                    // mq/mod.rs:133-155
         "s_sub_u32 %[k], %[bp], %[db]\n\t"
         "s_add_u32 %[k], %[k], 1\n\t"
         "v_readlane_b32 %[n0], %[w0], %[k]\n\t"
@@ -181,30 +184,30 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_cmp_lt_u32 %[k], 64\n\t"
         "s_cselect_b32 %[n0], %[n0], %[n1]\n\t"
         "s_cmp_eq_u32 %[cur], 0xff\n\t"
-        "s_cbranch_scc0 .Lbplain%=\n\t"
+        "s_cbranch_scc0 6f\n\t"
         "s_cmp_gt_u32 %[n0], 0x8f\n\t"
-        "s_cbranch_scc0 .Lbstuff%=\n\t"
+        "s_cbranch_scc0 5f\n\t"
         "s_add_u32 %[c], %[c], 0xff00\n\t"
         "s_mov_b32 %[ct], 8\n\t"
-        "s_branch .Lrn2%=\n"
-        ".Lbstuff%=:\n\t"
+        "s_branch 3b\n"
+        "5:\n\t"
         "s_add_u32 %[bp], %[bp], 1\n\t"
         "s_mov_b32 %[cur], %[n0]\n\t"
         "s_lshl_b32 %[n0], %[n0], 9\n\t"
         "s_add_u32 %[c], %[c], %[n0]\n\t"
         "s_mov_b32 %[ct], 7\n\t"
-        "s_branch .Lrn2%=\n"
-        ".Lbplain%=:\n\t"
+        "s_branch 3b\n"
+        "6:\n\t"
         "s_add_u32 %[bp], %[bp], 1\n\t"
         "s_mov_b32 %[cur], %[n0]\n\t"
         "s_lshl_b32 %[n0], %[n0], 8\n\t"
         "s_add_u32 %[c], %[c], %[n0]\n\t"
         "s_mov_b32 %[ct], 8\n\t"
-        "s_branch .Lrn2%=\n"
-        ".Lup%=:\n\t"                        // upper sub-interval, a < 0x8000: LPS iff a < qe
+        "s_branch 3b\n"
+        "7:\n\t"                        // upper sub-interval, a < 0x8000: LPS iff a < qe
         "s_cmp_lt_u32 %[a], %[q]\n\t"
         "s_cselect_b32 %[k], -1, 0\n"
-        ".Lcommon%=:\n\t"                    // k = all ones on LPS
+        "8:\n\t"                    // k = all ones on LPS
         "s_xor_b32 %[bit], %[e], %[k]\n\t"   // bit 31: MPS ^ LPS = the decision
         "s_lshr_b32 %[bit], %[bit], 31\n\t"
         "s_lshl_b32 %[n0], %[e], 3\n\t"
@@ -220,11 +223,11 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_flbit_i32_b32 %[t], %[a]\n\t"
         "s_sub_u32 %[t], %[t], 16\n\t"
         "s_cmp_gt_u32 %[t], %[ct]\n\t"
-        "s_cbranch_scc1 .Lrn%=\n\t"
+        "s_cbranch_scc1 2b\n\t"
         "s_lshl_b32 %[a], %[a], %[t]\n\t"
         "s_lshl_b32 %[c], %[c], %[t]\n\t"
         "s_sub_u32 %[ct], %[ct], %[t]\n"
-        ".Ldone%=:"
+        "9:"
         : [a] "+s"(d.a), [c] "+s"(d.c), [ct] "+s"(d.ct), [bp] "+s"(d.bp), [cur] "+s"(d.cur), [m0] "+{m0}"(ctx), [bit] "=&s"(bit), [e] "=&s"(e), [q] "=&s"(q), [t] "=&s"(t), [k] "=&s"(k),
           [n0] "=&s"(n0), [n1] "=&s"(n1)
         : [tab] "v"(d.tab), [w0] "v"(d.w0), [w1] "v"(d.w1), [db] "s"(d.dbase), [f16] "s"(0x60010u)
@@ -273,7 +276,7 @@ struct Coder<true> {
 template <bool DEC>
 __device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t negs, uint64_t& plane,
                             int w, int h) {
-    const int lane = threadIdx.x;
+    const int lane = lid();
     uint32_t newsig = 0;
     const int nstripes = (h + 3) >> 2;
     for (int s = 0; s < nstripes; s++) {
@@ -343,7 +346,7 @@ __device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
 // loop unrolls and a decision's context is one bit-field extract.
 template <bool DEC>
 __device__ void mr_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t& plane, int w, int h) {
-    const int lane = threadIdx.x;
+    const int lane = lid();
     const int nstripes = (h + 3) >> 2;
     for (int s = 0; s < nstripes; s++) {
         const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
@@ -387,7 +390,7 @@ __device__ void mr_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits,
 template <bool DEC>
 __device__ uint32_t cl_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t negs, uint64_t& plane,
                             int w, int h) {
-    const int lane = threadIdx.x;
+    const int lane = lid();
     uint32_t newsig = 0;
     const int nstripes = (h + 3) >> 2;
     for (int s = 0; s < nstripes; s++) {
@@ -485,7 +488,7 @@ __device__ uint32_t cl_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
 }
 
 __device__ __forceinline__ void load_tables(Tables& t, const uint32_t* __restrict__ tabs, int band) {
-    const int lane = threadIdx.x;
+    const int lane = lid();
     t.mq = tabs[lane];
     t.zc = tabs[64 + band * 64 + lane];
     t.sc = tabs[320 + lane];
@@ -577,7 +580,7 @@ __device__ __forceinline__ uint32_t sc_sym(const Tables& t, uint32_t si, uint32_
 // iterating lane-parallel; each round moves information one column east.
 __device__ __forceinline__ uint32_t form_spp(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
                              uint32_t& newsig) {
-    const bool valid = (int)threadIdx.x < w;
+    const bool valid = (int)lid() < w;
     const int nstripes = (h + 3) >> 2;
     uint32_t total = 0, ns = 0;
     for (int s = 0; s < nstripes; s++) {
@@ -636,7 +639,7 @@ __device__ __forceinline__ uint32_t form_spp(const Tables& t, Col& st, uint64_t 
 // significant samples not coded in this plane's SP pass; the context is 16
 // once refined, else 15/14 by any significant 8-neighbour.
 __device__ __forceinline__ uint32_t form_mrp(Col& st, uint64_t bits, int w, int h, uint8_t* symb) {
-    const bool valid = (int)threadIdx.x < w;
+    const bool valid = (int)lid() < w;
     const int nstripes = (h + 3) >> 2;
     uint32_t total = 0;
     for (int s = 0; s < nstripes; s++) {
@@ -669,7 +672,7 @@ __device__ __forceinline__ uint32_t form_mrp(Col& st, uint64_t bits, int w, int 
 // set the run length as two UNI decisions and the first sample's sign.
 __device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t bits, uint64_t negs, int w, int h, uint8_t* symb,
                              uint32_t& newsig) {
-    const bool valid = (int)threadIdx.x < w;
+    const bool valid = (int)lid() < w;
     const int nstripes = (h + 3) >> 2;
     uint32_t total = 0, ns = 0;
     for (int s = 0; s < nstripes; s++) {
@@ -753,47 +756,47 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_lshl_b32 %[t], %[a], 16\n\t"
         "s_andn2_b32 %[t], %[t], %[u]\n\t"
         "s_cmp_lt_i32 %[t], 0\n\t"
-        "s_cbranch_scc0 .Lslow%=\n\t"       // renormalising path: the one taken branch
+        "s_cbranch_scc0 7f\n\t"       // renormalising path: the one taken branch
         "s_add_u32 %[c], %[c], %[q]\n\t"
-        "s_branch .Ldone%=\n"
-        ".Lbyte%=:\n\t"                      // t >= ct: shift by ct and move a byte out
+        "s_branch 9f\n"
+        "1:\n\t"                      // t >= ct: shift by ct and move a byte out
         "s_lshl_b32 %[a], %[a], %[ct]\n\t"
         "s_lshl_b32 %[c], %[c], %[ct]\n\t"
         "s_sub_u32 %[t], %[t], %[ct]\n\t"
         "s_cmp_eq_u32 %[last], 0xff\n\t"
-        "s_cbranch_scc1 .Lbff%=\n\t"
+        "s_cbranch_scc1 2f\n\t"
         "s_bitcmp1_b32 %[c], 27\n\t"
-        "s_cbranch_scc0 .Lbplain%=\n\t"
+        "s_cbranch_scc0 3f\n\t"
         "s_add_u32 %[last], %[last], 1\n\t"
         "s_cmp_eq_u32 %[last], 0xff\n\t"
-        "s_cbranch_scc0 .Lbplain%=\n\t"
+        "s_cbranch_scc0 3f\n\t"
         "s_bitset0_b32 %[c], 27\n"
-        ".Lbff%=:\n\t"
+        "2:\n\t"
         "s_lshr_b32 %[q], %[c], 20\n\t"
         "s_and_b32 %[c], %[c], 0xfffff\n\t"
         "s_mov_b32 %[ct], 7\n\t"
-        "s_branch .Lemit%=\n"
-        ".Lbplain%=:\n\t"
+        "s_branch 4f\n"
+        "3:\n\t"
         "s_lshr_b32 %[q], %[c], 19\n\t"
         "s_and_b32 %[c], %[c], 0x7ffff\n\t"
         "s_mov_b32 %[ct], 8\n"
         // commit the held byte at window index pos - 1 - wbase, hold the new one
-        ".Lemit%=:\n\t"
+        "4:\n\t"
         "s_cmp_eq_u32 %[pos], 0\n\t"
-        "s_cbranch_scc1 .Lefirst%=\n\t"
+        "s_cbranch_scc1 6f\n\t"
         "s_sub_u32 %[u], %[pos], %[wb]\n\t"
         "s_sub_u32 m0, %[u], 1\n\t"
         "s_cmp_lt_u32 m0, 64\n\t"
-        "s_cbranch_scc0 .Lehi%=\n\t"
+        "s_cbranch_scc0 5f\n\t"
         "v_writelane_b32 %[w0], %[last], m0\n\t"
-        "s_branch .Lefirst%=\n"
-        ".Lehi%=:\n\t"
+        "s_branch 6f\n"
+        "5:\n\t"
         "v_writelane_b32 %[w1], %[last], m0\n"
-        ".Lefirst%=:\n\t"
+        "6:\n\t"
         "s_add_u32 %[pos], %[pos], 1\n\t"
         "s_and_b32 %[last], %[q], 0xff\n\t"
-        "s_branch .Lrn%=\n"
-        ".Lslow%=:\n\t"
+        "s_branch 8f\n"
+        "7:\n\t"
         // renormalising decision, MPS or LPS, branch-free: conditional
         // exchange to_qe = (a < qe) XOR lps; MPS: a = qe if a < qe else
         // c += qe; LPS: c += qe if a < qe else a = qe; state NMPS / NLPS,
@@ -818,13 +821,13 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_flbit_i32_b32 %[t], %[a]\n\t"
         "s_sub_u32 %[t], %[t], 16\n"
         // renormalise by t bits, a byte out whenever ct reaches zero
-        ".Lrn%=:\n\t"
+        "8:\n\t"
         "s_cmp_lt_u32 %[t], %[ct]\n\t"
-        "s_cbranch_scc0 .Lbyte%=\n\t"
+        "s_cbranch_scc0 1b\n\t"
         "s_lshl_b32 %[a], %[a], %[t]\n\t"
         "s_lshl_b32 %[c], %[c], %[t]\n\t"
         "s_sub_u32 %[ct], %[ct], %[t]\n"
-        ".Ldone%=:"
+        "9:"
         : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
           [cx] "+v"(m.cx), [w0] "+v"(m.w0), [w1] "+v"(m.w1), [m0] "+{m0}"(ctx), [e] "=&s"(e), [q] "=&s"(q),
           [u] "=&s"(u), [t] "=&s"(t)
@@ -834,7 +837,7 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
 // Store the window's lower 64 bytes once they are all committed.
 __device__ __forceinline__ void mqe_spill(MqE& m) {
     if (m.pos > m.wbase + 64u) {
-        if (m.wbase + 64u <= m.cap) m.out[m.wbase + threadIdx.x] = (uint8_t)m.w0;
+        if (m.wbase + 64u <= m.cap) m.out[m.wbase + lid()] = (uint8_t)m.w0;
         else m.overflow = 1u;
         m.w0 = m.w1;
         m.wbase += 64u;
@@ -843,7 +846,7 @@ __device__ __forceinline__ void mqe_spill(MqE& m) {
 // Code the n symbol bytes of one pass from LDS.
 __device__ __forceinline__ void mqe_pass(MqE& m, const uint32_t* symw, uint32_t n) {
     for (uint32_t base = 0; base < n; base += 256u) {
-        const uint32_t sv = symw[(base >> 2) + threadIdx.x];
+        const uint32_t sv = symw[(base >> 2) + lid()];
         const uint32_t cnt = min(256u, n - base);
         uint32_t q = 0;
         for (; q + 4u <= cnt; q += 4u) {
@@ -911,7 +914,7 @@ __device__ __forceinline__ uint32_t mqe_flush(MqE& m) {
         else m.w1 = wl(m.w1, m.last, idx - 64u);
     }
     const uint32_t nwin = m.pos - m.wbase;  // <= 128 bytes pending
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = lid();
     if (lane < nwin) {
         const uint32_t o = m.wbase + lane;
         if (o < m.cap) m.out[o] = (uint8_t)m.w0;
@@ -932,7 +935,7 @@ __device__ __forceinline__ uint32_t mqe_flush(MqE& m) {
 // t1.rs:215-258: lane x gets column x as row magnitudes and a negative mask.
 // Returns mbp (0 for an all-zero block).
 __device__ __forceinline__ int load_block(const BlockJob& J, uint32_t (&mag)[64], uint64_t& neg, BlockResult& R) {
-    const int lane = threadIdx.x;
+    const int lane = lid();
     const int w = J.w, h = J.h;
     const bool lossy = J.step > 0.0f;
     neg = 0;
@@ -976,7 +979,7 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
                                                        uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs) {
     __shared__ uint32_t symw[kSymCap / 4];
     uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
-    const int lane = threadIdx.x;
+    const int lane = lid();
     const BlockJob J = jobs[blockIdx.x];
     const int w = J.w, h = J.h;
     uint32_t mag[64];
@@ -1095,7 +1098,7 @@ __global__ __launch_bounds__(64) void t1_form_kernel(const BlockJob* __restrict_
                                                      unsigned long long pool_cap) {
     __shared__ uint32_t symw[kSymCap / 4];
     uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
-    const int lane = threadIdx.x;
+    const int lane = lid();
     const BlockJob J = jobs[blockIdx.x];
     const int w = J.w, h = J.h;
     uint32_t mag[64];
@@ -1217,7 +1220,7 @@ __global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __res
                                                    uint8_t* __restrict__ out) {
     __shared__ uint32_t cst[32 * 64];  // 19 contexts used; sized for any 5-bit context code
     __shared__ uint32_t tab[64];
-    const int lane = threadIdx.x;
+    const int lane = lid();
     tab[lane] = tabs[lane];
     __syncthreads();
 #pragma unroll
@@ -1324,7 +1327,7 @@ __global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __res
 // i32 for 5/3.  Blocks that are not included are written as zeros.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap))) void t1_decode_kernel(const DecodeJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                        const uint8_t* __restrict__ payload) {
-    const int lane = threadIdx.x;
+    const int lane = lid();
     const DecodeJob J = jobs[blockIdx.x];
     const int w = __builtin_amdgcn_readfirstlane(J.w), h = __builtin_amdgcn_readfirstlane(J.h);
     const bool lossy = J.step > 0.0f;
