@@ -1,6 +1,7 @@
 #include "server.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace j2l {
 
@@ -11,6 +12,31 @@ constexpr size_t kEncBytesPerSample = 48;
 // Decode: payload, coefficient plane, LL ping-pong, output.
 constexpr size_t kDecBytesPerSample = 16;
 }  // namespace
+
+PinnedPool::~PinnedPool() {
+    for (auto& kv : free_) (void)hipHostFree(kv.second);
+}
+
+uint8_t* PinnedPool::get(size_t n) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.lower_bound(n);
+    if (it != free_.end() && it->first <= 2 * n) {  // reuse a buffer of a similar size
+        uint8_t* p = it->second;
+        free_.erase(it);
+        return p;
+    }
+    if (held_ + n > cap_) return nullptr;
+    uint8_t* p = nullptr;
+    if (hipHostMalloc((void**)&p, n, hipHostMallocDefault) != hipSuccess) return nullptr;
+    held_ += n;
+    return p;
+}
+
+void PinnedPool::put(uint8_t* p, size_t n) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    free_.emplace(n, p);
+}
 
 Server::Server(int slots, size_t mem_budget_bytes, int max_batch) {
     if (slots < 1 || slots > 8) throw Error(INVALID_INPUT, "server slots must be 1..8");
@@ -58,9 +84,17 @@ uint64_t Server::submit_encode(const uint8_t* pixels, uint32_t w, uint32_t h, ui
         r->enc.src_kind = SRC_DEV_U8;
         r->enc.src = pixels;
     } else {
-        r->pixels.assign(pixels, pixels + n);
         r->enc.src_kind = SRC_HOST_U8;
-        r->enc.src = r->pixels.data();
+        r->pinned = pinned_.get(n);
+        if (r->pinned) {
+            // a pooled buffer can be up to twice n; the request records n
+            r->pinned_size = n;
+            std::memcpy(r->pinned, pixels, n);
+            r->enc.src = r->pinned;
+        } else {
+            r->pixels.assign(pixels, pixels + n);
+            r->enc.src = r->pixels.data();
+        }
     }
     r->est_bytes = n * kEncBytesPerSample;
     return enqueue(r);
@@ -164,6 +198,12 @@ void Server::worker(int slot) {
             for (auto* r : dr) r->dout.status = IO, r->dout.message = e.what();
         }
         const auto t1 = std::chrono::steady_clock::now();
+        for (auto& r : batch) {  // the batch has finished with its inputs
+            if (r->pinned) {
+                pinned_.put(r->pinned, r->pinned_size);
+                r->pinned = nullptr;
+            }
+        }
         {
             std::lock_guard<std::mutex> lk(mu_);
             for (auto& r : batch) {

@@ -13,6 +13,7 @@
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -23,11 +24,29 @@
 
 namespace j2l {
 
+// Pinned (page-locked) host buffers for submitted pixels, recycled across
+// requests, so an encode's upload is one DMA instead of a staged copy.
+// Beyond `cap` bytes in use, requests fall back to pageable memory.
+class PinnedPool {
+public:
+    explicit PinnedPool(size_t cap) : cap_(cap) {}
+    ~PinnedPool();
+    uint8_t* get(size_t n);  // nullptr when over the cap or out of memory
+    void put(uint8_t* p, size_t n);
+
+private:
+    std::mutex mu_;
+    std::multimap<size_t, uint8_t*> free_;
+    size_t held_ = 0, cap_;
+};
+
 struct Request {
     uint64_t id = 0;
     bool is_encode = true;
     EncItem enc;
-    std::vector<uint8_t> pixels;  // host pixels copied at submission
+    uint8_t* pinned = nullptr;    // host pixels copied at submission (pinned pool) ...
+    size_t pinned_size = 0;
+    std::vector<uint8_t> pixels;  // ... or pageable when the pool is full
     std::vector<uint8_t> stream;  // decode input copied at submission
     size_t est_bytes = 0;         // device bytes the request is expected to hold
     std::chrono::steady_clock::time_point t_submit, t_start, t_done;
@@ -68,6 +87,7 @@ private:
     std::vector<std::thread> threads_;
     bool stop_ = false;
     uint64_t next_id_ = 1;
+    PinnedPool pinned_{size_t(4) << 30};
     size_t budget_per_slot_;
     int max_batch_;
     ServerStats stats_;
