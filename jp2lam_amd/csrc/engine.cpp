@@ -445,24 +445,46 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     }
 
     // ---- uploads ---------------------------------------------------------------------
+    // The descriptor tables go through the page-locked arena, so each copy is
+    // one asynchronous DMA (a pageable source makes the runtime stage it
+    // synchronously); the same arena then receives the readback.
     hipStream_t st = stream_;
     auto up = [&](size_t off, const void* src, size_t bytes) {
         if (bytes) J2L_HIP(hipMemcpyAsync(D + off, src, bytes, hipMemcpyHostToDevice, st));
     };
+    Layout H;
+    struct Staged {
+        size_t dst, hoff, n;
+        const void* src;
+    };
+    std::vector<Staged> staged;
+    auto stage = [&](size_t off, const void* src, size_t bytes) {
+        if (bytes) staged.push_back({off, H.add(bytes), bytes, src});
+    };
+    std::vector<unsigned long long> gmax0(ngroups, 0ull);
+    stage(o_jobs, jobs.data(), sizeof(BlockJob) * nb);
+    stage(o_pjobs, pjobs.data(), sizeof(PcrdJob) * nb);
+    stage(o_gmax, gmax0.data(), 8 * ngroups);
+    stage(o_rawf, raw_floor.data(), 16 * ngroups);
+    stage(o_imgfirst, img_first.data(), 4 * img_first.size());
+    stage(o_blkimg, blk_img.data(), 4 * nb);
+    stage(o_packbase, pack_base.data(), 8 * imgs.size());
+    for (int kind = 0; kind < 2; kind++) {
+        stage(o_pseg_k[kind], pseg_k[kind].data(), sizeof(PixSeg) * pseg_k[kind].size());
+        for (int k = 0; k < maxL; k++) stage(o_fseg_k[kind][k], fseg_k[kind][k].data(), sizeof(DwtSeg) * fseg_k[kind][k].size());
+    }
+    const size_t h_sel = H.add(sizeof(Selection) * nb);
+    const size_t h_res = H.add(sizeof(BlockResult) * nb);
+    const size_t h_tot = H.add(4 * imgs.size());
+    const size_t h_lam = H.add(8 * ngroups);
+    pin_.reserve(H.used);  // once per call: DMAs from it stay in flight
+    char* P = pin_.base;
+    for (auto& u : staged) {
+        std::memcpy(P + u.hoff, u.src, u.n);
+        J2L_HIP(hipMemcpyAsync(D + u.dst, P + u.hoff, u.n, hipMemcpyHostToDevice, st));
+    }
     for (auto& im : imgs)
         if (im.in_bytes) up(im.in_off, im.src, im.in_bytes);
-    std::vector<unsigned long long> gmax0(ngroups, 0ull);
-    up(o_jobs, jobs.data(), sizeof(BlockJob) * nb);
-    up(o_pjobs, pjobs.data(), sizeof(PcrdJob) * nb);
-    up(o_gmax, gmax0.data(), 8 * ngroups);
-    up(o_rawf, raw_floor.data(), 16 * ngroups);
-    up(o_imgfirst, img_first.data(), 4 * img_first.size());
-    up(o_blkimg, blk_img.data(), 4 * nb);
-    up(o_packbase, pack_base.data(), 8 * imgs.size());
-    for (int kind = 0; kind < 2; kind++) {
-        up(o_pseg_k[kind], pseg_k[kind].data(), sizeof(PixSeg) * pseg_k[kind].size());
-        for (int k = 0; k < maxL; k++) up(o_fseg_k[kind][k], fseg_k[kind][k].data(), sizeof(DwtSeg) * fseg_k[kind][k].size());
-    }
 
     // ---- device pipeline -------------------------------------------------------------
     Events ev;
@@ -535,13 +557,6 @@ t1_again:
     J2L_HIP(hipEventRecord(ev.e[3], st));
 
     // ---- readback ----------------------------------------------------------------------
-    Layout H;
-    const size_t h_sel = H.add(sizeof(Selection) * nb);
-    const size_t h_res = H.add(sizeof(BlockResult) * nb);
-    const size_t h_tot = H.add(4 * imgs.size());
-    const size_t h_lam = H.add(8 * ngroups);
-    pin_.reserve(H.used);
-    char* P = pin_.base;
     J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, sizeof(Selection) * nb, hipMemcpyDeviceToHost, st));
     J2L_HIP(hipMemcpyAsync(P + h_res, D + o_res, sizeof(BlockResult) * nb, hipMemcpyDeviceToHost, st));
     J2L_HIP(hipMemcpyAsync(P + h_tot, D + o_totals, 4 * imgs.size(), hipMemcpyDeviceToHost, st));
@@ -589,12 +604,16 @@ t1_again:
     }
     size_t packed_bytes = 0;
     for (size_t k = 0; k < imgs.size(); k++) packed_bytes += totv[k];
-    std::vector<uint8_t> packed_host(packed_bytes);
+    // The selected bytes come back into the page-locked arena (one DMA per
+    // image, no staging).  The synchronisation above has retired every copy
+    // from the arena, and its readback is already copied out.
+    pin_.reserve(packed_bytes);
+    const uint8_t* packed_host = (const uint8_t*)pin_.base;
     {
         size_t o = 0;
         for (size_t k = 0; k < imgs.size(); k++) {
             if (totv[k])
-                J2L_HIP(hipMemcpyAsync(packed_host.data() + o, D + o_packed + pack_base[k], totv[k], hipMemcpyDeviceToHost, st));
+                J2L_HIP(hipMemcpyAsync(pin_.base + o, D + o_packed + pack_base[k], totv[k], hipMemcpyDeviceToHost, st));
             o += totv[k];
         }
     }
@@ -619,7 +638,7 @@ t1_again:
         Img& im = imgs[k];
         EncOut& out = outs[im.idx];
         const Plan& p = im.plan;
-        const uint8_t* body = packed_host.data() + pk;
+        const uint8_t* body = packed_host + pk;
         pk += totv[k];
         bool overflow = false;
         for (uint32_t b = im.blk_first; b < im.blk_first + im.blk_count; b++) overflow |= resv[b].overflow != 0;
@@ -944,8 +963,18 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     }
 
     hipStream_t st = stream_;
-    for (auto& im : imgs)
-        J2L_HIP(hipMemcpyAsync(D + o_pay + im.pay_off, im.ci.payload, im.ci.payload_len, hipMemcpyHostToDevice, st));
+    // payloads and tables go through the page-locked arena (one asynchronous
+    // DMA each; a pageable source makes the runtime stage it synchronously)
+    Layout H;
+    struct Staged {
+        size_t dst, hoff, n;
+        const void* src;
+    };
+    std::vector<Staged> staged;
+    auto stage = [&](size_t off, const void* src, size_t bytes) {
+        if (bytes) staged.push_back({off, H.add(bytes), bytes, src});
+    };
+    for (auto& im : imgs) stage(o_pay + im.pay_off, im.ci.payload, im.ci.payload_len);
     if (nj) {
         // issue priority by segment length rank: the kernel ends with its
         // heaviest block, so those blocks win their SIMD's issue slots
@@ -958,15 +987,16 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         const uint32_t t1 = sorted[std::min<size_t>(nj - 1, (size_t)nj * 3 / 10)];
         for (int k = 0; k < nj; k++)
             jobs[k].pad = (uint8_t)(lens[k] >= t3 && lens[k] ? 3 : lens[k] >= t2 && lens[k] ? 2 : lens[k] >= t1 && lens[k] ? 1 : 0);
-        J2L_HIP(hipMemcpyAsync(D + o_jobs, jobs.data(), sizeof(DecodeJob) * nj, hipMemcpyHostToDevice, st));
+        stage(o_jobs, jobs.data(), sizeof(DecodeJob) * nj);
     }
-    for (size_t q = 0; q < isegs.size(); q++)
-        if (!isegs[q].empty())
-            J2L_HIP(hipMemcpyAsync(D + o_iseg[q], isegs[q].data(), sizeof(DwtSeg) * isegs[q].size(), hipMemcpyHostToDevice, st));
-    if (!fsegs[0].empty())
-        J2L_HIP(hipMemcpyAsync(D + o_fseg0, fsegs[0].data(), sizeof(PixSeg) * fsegs[0].size(), hipMemcpyHostToDevice, st));
-    if (!fsegs[1].empty())
-        J2L_HIP(hipMemcpyAsync(D + o_fseg1, fsegs[1].data(), sizeof(PixSeg) * fsegs[1].size(), hipMemcpyHostToDevice, st));
+    for (size_t q = 0; q < isegs.size(); q++) stage(o_iseg[q], isegs[q].data(), sizeof(DwtSeg) * isegs[q].size());
+    stage(o_fseg0, fsegs[0].data(), sizeof(PixSeg) * fsegs[0].size());
+    stage(o_fseg1, fsegs[1].data(), sizeof(PixSeg) * fsegs[1].size());
+    pin_.reserve(H.used);
+    for (auto& u : staged) {
+        std::memcpy(pin_.base + u.hoff, u.src, u.n);
+        J2L_HIP(hipMemcpyAsync(D + u.dst, pin_.base + u.hoff, u.n, hipMemcpyHostToDevice, st));
+    }
 
     Events ev;
     J2L_HIP(hipEventRecord(ev.e[0], st));
