@@ -51,6 +51,9 @@ def enc_plain_host(j, px, q):
     return j.encode_u8(px, q)
 
 
+CODERS = {1: "wave per block", 2: "lane per block", 4: "hybrid (long streams scalar, short on lanes)"}
+
+
 def level_areas(w, h, levels):
     areas = []
     for _ in range(levels):
@@ -188,7 +191,7 @@ def main():
         batch_ms = (time.perf_counter() - t0) * 1000.0 / breps
         tb = np.zeros(4, np.float64)
         lib.jp2g_last_timings(tb.ctypes.data)
-        batched = {"pages": nb, "t1_coder": "lane per block" if lib.jp2g_last_t1_lanes() else "wave per block",
+        batched = {"pages": nb, "t1_coder": CODERS.get(lib.jp2g_last_t1_coder()),
                    "encode_ms_per_batch": round(batch_ms, 3),
                    "encode_mpx_s": round(nb * W * H / 1e6 / (batch_ms / 1e3), 2),
                    "plain_encode_ms_per_page": round(plain_ms, 3),
@@ -250,6 +253,7 @@ def main():
             torch.cuda.synchronize()
             burst(True)
             sdone, t_sat = burst(True)
+            burst(False)  # fills the pinned pool
             hdone, t_sat_h = burst(False)
         lat = np.array([r.total_ms for r in done])
         ok = all(done[i].value == want[i % len(pool)] for i in range(n))
@@ -308,7 +312,7 @@ def main():
         enc3, e3 = timed(lambda: j.encode_many([((dph.data_ptr(), H3, W3, 3), 75)], device=True)[0], max(3, args.steps // 2))
         te3 = np.zeros(4, np.float64)
         lib.jp2g_last_timings(te3.ctypes.data)
-        coder3 = "lane per block" if lib.jp2g_last_t1_lanes() else "wave per block"
+        coder3 = CODERS.get(lib.jp2g_last_t1_coder())
         _, d3 = timed(lambda: j.decode_many([enc3], device_outputs=[(dph_out.data_ptr(), dph_out.numel())])[0],
                       max(3, args.steps // 2))
         td3 = np.zeros(4, np.float64)

@@ -115,6 +115,8 @@ SIGNATURES = [
     ("jp2g_set_debug", None, [C.c_int]),
     ("jp2g_set_t1_mode", None, [C.c_int, C.c_uint32]),
     ("jp2g_last_t1_lanes", C.c_int, []),
+    ("jp2g_last_t1_coder", C.c_int, []),
+    ("jp2g_set_hybrid_frac", None, [C.c_float]),
     ("jp2g_last_block_count", C.c_int, []),
     ("jp2g_last_block_info", None, [C.c_int, C.c_void_p]),
     ("jp2g_last_block_passes", None, [C.c_int, C.c_void_p]),

@@ -156,6 +156,11 @@ void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResu
                     hipStream_t st);
 void launch_t1_mq(const BlockJob* jobs, int n, const uint32_t* tabs, const uint8_t* pool, const uint32_t* soff16,
                   const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, hipStream_t st);
+// Long streams on the scalar coder, short ones on lanes (after launch_t1_form);
+// lists holds 2 + 2n words.
+void launch_t1_hybrid(const BlockJob* jobs, int n, const uint32_t* tabs, const uint8_t* pool, const uint32_t* soff16,
+                      const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, uint32_t* lists, float frac,
+                      hipStream_t st);
 uint64_t t1_stream_bound(uint32_t mbp);
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st);
 void launch_pcrd(int n, bool lossless, int ngroups, const BlockResult* res, const uint32_t* snaps,

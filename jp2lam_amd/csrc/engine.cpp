@@ -27,7 +27,7 @@ struct Layout {
 };
 
 struct Events {
-    hipEvent_t e[6];
+    hipEvent_t e[8];
     Events() {
         for (auto& x : e) J2L_HIP(hipEventCreate(&x));
     }
@@ -278,17 +278,23 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     // coders on the same coefficients and keeps the faster; smaller batches
     // use the wave coder.
     const int bucket = nb ? 32 - __builtin_clz((unsigned)nb) : 0;
-    int coder = t1_mode == 1 || t1_mode == 2 ? t1_mode : 1;
+    int coder = t1_mode == 1 || t1_mode == 2 || t1_mode == 4 ? t1_mode : 1;
+    float frac = hybrid_frac;
     bool probe = false;
     if (t1_mode == 3) {
         coder = (uint32_t)nb >= lane_min_blocks ? 2 : 1;
     } else if (t1_mode == 0 && (uint32_t)nb >= lane_min_blocks) {
         auto it = tuned_.find(bucket);
-        if (it != tuned_.end()) coder = it->second;
-        else probe = true;
+        if (it != tuned_.end()) {
+            coder = it->second.coder;
+            frac = it->second.frac;
+        } else {
+            probe = true;
+        }
     }
-    const bool lanes = coder == 2 || probe;
-    size_t o_pend = 0, o_soff = 0, o_pused = 0, o_pool = 0;
+    static constexpr float kProbeFrac[2] = {0.55f, 0.75f};
+    const bool lanes = coder != 1 || probe;  // lane (2) and hybrid (4) code formed streams
+    size_t o_pend = 0, o_soff = 0, o_pused = 0, o_pool = 0, o_lists = 0;
     unsigned long long pool_cap = 0;
     if (lanes) {
         for (auto& j : jobs) pool_cap += t1_stream_bound((uint32_t)j.mb + 2u);
@@ -296,6 +302,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         o_soff = L.add(sizeof(uint32_t) * nb);
         o_pused = L.add(8);
         o_pool = L.add(pool_cap + 64);
+        o_lists = L.add(4 * (2 + 2 * (size_t)nb));
     }
 
     // Segment tables, split by sample type (f32 for 9/7, i32 for 5/3) so
@@ -504,25 +511,39 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
             for (int c = 0; c < im.plan.ncomp; c++)
                 J2L_HIP(hipMemcpyAsync(D + im.p_off[c], D + im.x_off[c], im.n * 4, hipMemcpyDeviceToDevice, st));
     const unsigned long long zero64 = 0;
-    bool use_lanes = lanes;
-    if (probe) {
-        // wave coder timed first; the lane coder then runs below and its
-        // (identical) results are the ones used
-        J2L_HIP(hipEventRecord(ev.e[4], st));
-        launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
-                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
-        J2L_HIP(hipEventRecord(ev.e[5], st));
-    }
-t1_again:
-    J2L_HIP(hipEventRecord(ev.e[1], st));
-    if (use_lanes) {
+    // formed-stream coders: 2 = lane per block, 4 = hybrid
+    auto run_streams = [&](int which, float split) {
         J2L_HIP(hipMemcpyAsync(D + o_pused, &zero64, 8, hipMemcpyHostToDevice, st));
         launch_t1_form((const BlockJob*)(D + o_jobs), nb, tables_, (BlockResult*)(D + o_res), (uint32_t*)(D + o_pend),
                        (uint16_t*)(D + o_ns), (uint32_t*)(D + o_soff), (uint8_t*)(D + o_pool),
                        (unsigned long long*)(D + o_pused), pool_cap, st);
-        launch_t1_mq((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
-                     (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
-                     (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), st);
+        if (which == 2)
+            launch_t1_mq((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
+                         (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
+                         (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), st);
+        else
+            launch_t1_hybrid((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
+                             (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
+                             (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), (uint32_t*)(D + o_lists), split, st);
+    };
+    int run = probe ? 2 : coder;
+    if (probe) {
+        // the wave coder and the hybrid at both split points are timed
+        // first; the lane coder then runs below and its (identical) results
+        // are the ones used
+        J2L_HIP(hipEventRecord(ev.e[4], st));
+        launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
+                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
+        J2L_HIP(hipEventRecord(ev.e[5], st));
+        run_streams(4, kProbeFrac[0]);
+        J2L_HIP(hipEventRecord(ev.e[6], st));
+        run_streams(4, kProbeFrac[1]);
+        J2L_HIP(hipEventRecord(ev.e[7], st));
+    }
+t1_again:
+    J2L_HIP(hipEventRecord(ev.e[1], st));
+    if (run != 1) {
+        run_streams(run, frac);
     } else {
         launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
                          (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
@@ -567,19 +588,27 @@ t1_again:
     t_pcrd_ms = ev.ms(2, 3);
     const Selection* sel = (const Selection*)(P + h_sel);
     const BlockResult* res = (const BlockResult*)(P + h_res);
-    if (use_lanes) {
+    if (run != 1) {
         bool pool_full = false;
         for (int b = 0; b < nb && !pool_full; b++) pool_full = res[b].overflow == 2;
         if (pool_full) {
-            use_lanes = false;
+            run = 1;
             goto t1_again;
         }
     }
     if (probe) {
-        const float t_wave = ev.ms(4, 5), t_lane = ev.ms(1, 2);
-        tuned_[bucket] = use_lanes && t_lane < t_wave ? 2 : 1;
+        const float t_wave = ev.ms(4, 5), t_h0 = ev.ms(5, 6), t_h1 = ev.ms(6, 7), t_lane = ev.ms(1, 2);
+        T1Choice best{1, hybrid_frac};
+        if (run != 1) {  // the streams fitted the pool
+            float t = t_wave;
+            if (t_lane < t) t = t_lane, best = {2, hybrid_frac};
+            if (t_h0 < t) t = t_h0, best = {4, kProbeFrac[0]};
+            if (t_h1 < t) t = t_h1, best = {4, kProbeFrac[1]};
+        }
+        tuned_[bucket] = best;
     }
-    last_t1_lanes = use_lanes;
+    last_t1_lanes = run == 2;
+    last_t1_coder = run;
     const uint32_t* totals = (const uint32_t*)(P + h_tot);
     std::vector<Selection> selv(sel, sel + nb);
     std::vector<BlockResult> resv(res, res + nb);

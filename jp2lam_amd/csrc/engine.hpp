@@ -90,12 +90,22 @@ public:
     const std::vector<double>& last_lambdas() const { return last_lambdas_; }
     bool keep_debug = false;
     // Tier-1 encoder choice: 0 = tuned per batch-size class (classes from
-    // lane_min_blocks code-blocks up time both coders once), 1 = wave per
-    // block, 2 = lane per block, 3 = lane per block from lane_min_blocks up.
+    // lane_min_blocks code-blocks up time the coders once), 1 = wave per
+    // block, 2 = lane per block, 3 = lane per block from lane_min_blocks up,
+    // 4 = hybrid (streams longer than hybrid_frac x the longest on the
+    // scalar coder, the rest on lanes).
     int t1_mode = 0;
     uint32_t lane_min_blocks = 8192;
+    float hybrid_frac = 0.55f;
     bool last_t1_lanes = false;
-    std::map<int, int> tuned_;  // size class -> coder (1 wave, 2 lane)
+    int last_t1_coder = 1;
+    // size class -> coder (1 wave, 2 lane, 4 hybrid) and the hybrid's split
+    // point; the tuner tries the hybrid at two split points
+    struct T1Choice {
+        int coder;
+        float frac;
+    };
+    std::map<int, T1Choice> tuned_;
     // Stage timings of the last call (ms, from HIP events): prepare+DWT,
     // Tier-1, PCRD+pack, host Tier-2.
     double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;

@@ -1078,14 +1078,18 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
 // Stream region of a block with mbp bit-planes: per plane at most one
 // symbol per sample plus two extra per run-mode stripe-column (AGG=1 adds
 // AGG and two UNI and drops the samples before the run length), plus one
-// sign per sample, one pass marker per pass, and the reader's 32-byte
-// look-ahead; 16-byte aligned.
+// sign per sample, per pass one marker and up to three padding bytes, and
+// the reader's 32-byte look-ahead; 16-byte aligned.
 __host__ __device__ __forceinline__ uint64_t stream_bound(uint32_t mbp) {
-    return ((uint64_t)mbp * 6144u + 4096u + kMaxPasses + 32u + 15u) & ~(uint64_t)15u;
+    return ((uint64_t)mbp * 6144u + 4096u + 4u * kMaxPasses + 32u + 15u) & ~(uint64_t)15u;
 }
 // Pass marker: context 31, whose entry (Qe = 0, MPS 0) makes coding it a
-// no-op (a stays >= 0x8000, c += 0, no renormalisation).
+// no-op (a stays >= 0x8000, c += 0, no renormalisation).  Each pass's
+// region (symbols, padding, marker) is a multiple of 4 bytes, padded with
+// context 30, another no-op entry that records nothing, so a reader may take
+// a pass as whole words.
 constexpr uint32_t kMarker = 31u;
+constexpr uint32_t kPad = 30u;
 
 // Context formation for the lane-per-block coder: same passes as the fused
 // kernel, symbols copied from LDS to the block's stream in global memory,
@@ -1126,12 +1130,13 @@ __global__ __launch_bounds__(64) void t1_form_kernel(const BlockJob* __restrict_
     uint32_t np = 0, cum = 0;
     auto emit_pass = [&](uint32_t n, uint32_t ns) {
         __syncthreads();
-        fits = fits && cum + n + 1u + 32u <= bound;  // never past the region, whatever the bound's margin
+        const uint32_t len = (n + 4u) & ~3u;  // symbols, padding, marker
+        fits = fits && cum + len + 32u <= bound;  // never past the region, whatever the bound's margin
         if (fits) {
-            for (uint32_t i = lane; i < n; i += 64u) dst[cum + i] = symb[i];
-            if (lane == 0) dst[cum + n] = (uint8_t)kMarker;
+            for (uint32_t i = lane; i < len - 1u; i += 64u) dst[cum + i] = i < n ? symb[i] : (uint8_t)kPad;
+            if (lane == 0) dst[cum + len - 1u] = (uint8_t)kMarker;
         }
-        cum += n + 1u;
+        cum += len;
         if (np < 64u) {
             end0 = wlane(end0, cum, np);
             ns0 = wlane(ns0, ns, np);
@@ -1213,22 +1218,21 @@ __device__ __forceinline__ void lmq_byteout(LaneMq& m) {
 // step s while s < its stream length.  Each step is branch-free apart from
 // the byte output: the MPS-without-renormalisation case is the exchange
 // formula with the state written back unchanged and a zero shift.
-__global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __restrict__ jobs,
-                                                   const uint32_t* __restrict__ tabs, const uint8_t* __restrict__ pool,
-                                                   const uint32_t* __restrict__ soff16, const uint32_t* __restrict__ pend,
-                                                   BlockResult* __restrict__ results, uint32_t* __restrict__ snaps,
-                                                   uint8_t* __restrict__ out) {
-    __shared__ uint32_t cst[32 * 64];  // 19 contexts used; sized for any 5-bit context code
-    __shared__ uint32_t tab[64];
+// Lane coder: this lane codes block b (b < 0: none); cst is 32 x 64 words
+// and tab 64 words of LDS.
+__device__ __forceinline__ void lane_mq(int b, const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+                                        const uint8_t* __restrict__ pool, const uint32_t* __restrict__ soff16,
+                                        const uint32_t* __restrict__ pend, BlockResult* __restrict__ results,
+                                        uint32_t* __restrict__ snaps, uint8_t* __restrict__ out, uint32_t* cst,
+                                        uint32_t* tab) {
     const int lane = lid();
     tab[lane] = tabs[lane];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 19; k++)
         cst[k * 64 + lane] = tab[k == CTX_UNI ? 46 : k == CTX_AGG ? 3 : k == CTX_ZC ? 4 : 0];
-    const int b = blockIdx.x * 64 + lane;
     uint32_t np = 0, total = 0;
-    if (b < nb && results[b].overflow == 0) np = results[b].npasses;
+    if (b >= 0 && results[b].overflow == 0) np = results[b].npasses;
     const uint32_t* pe = pend + (size_t)b * kMaxPasses;
     if (np) total = pe[np - 1];
     const uint32_t smax = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_max(total));
@@ -1256,8 +1260,9 @@ __global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __res
         cur = src[0];
         nxt = src[1];
     }
-    // cst row kMarker stays 0: Qe 0, MPS 0
+    // rows kMarker and kPad: Qe 0, MPS 0 (no-op symbols)
     cst[kMarker * 64 + lane] = 0;
+    cst[kPad * 64 + lane] = 0;
     auto step = [&](uint32_t si, uint32_t sym) {
         sym = si < total ? sym & 0xffu : kMarker;  // past the end: no-op symbols
         if (si < total && (sym & 31u) == kMarker) sn[k++] = m.pos;  // end of a pass
@@ -1319,6 +1324,149 @@ __global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __res
     }
     results[b].nbytes = fin;
     results[b].overflow = m.ovf ? 1 : 0;
+}
+
+__global__ __launch_bounds__(64) void t1_mq_kernel(int nb, const BlockJob* __restrict__ jobs,
+                                                   const uint32_t* __restrict__ tabs, const uint8_t* __restrict__ pool,
+                                                   const uint32_t* __restrict__ soff16, const uint32_t* __restrict__ pend,
+                                                   BlockResult* __restrict__ results, uint32_t* __restrict__ snaps,
+                                                   uint8_t* __restrict__ out) {
+    __shared__ uint32_t cst[32 * 64];  // 19 contexts used; sized for any 5-bit context code
+    __shared__ uint32_t tab[64];
+    const int b = blockIdx.x * 64 + lid();
+    lane_mq(b < nb ? b : -1, jobs, tabs, pool, soff16, pend, results, snaps, out, cst, tab);
+}
+
+// ---- hybrid Tier-1 coder: long streams on the scalar coder, the rest on lanes --
+//
+// The lane coder's time is its longest stream at one vector step per symbol,
+// the scalar coder's is its block's chain at one scalar sequence per
+// symbol (about three times faster per symbol) but all its blocks share the
+// CUs' scalar units.  When a batch's streams are long but few, the longest
+// go to the scalar coder and the rest to lanes, in one launch, so both units
+// work.  The formation is t1_form_kernel's for every block.
+
+// Scalar coder over a formed stream (wave per block): the pass regions are
+// whole words, read 64 words (256 symbols) per load with the next load in
+// flight; markers and padding code as no-ops (their context lanes hold a
+// zero entry).
+__device__ __forceinline__ void stream_mq(int b, const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+                                          const uint8_t* __restrict__ pool, const uint32_t* __restrict__ soff16,
+                                          const uint32_t* __restrict__ pend, BlockResult* __restrict__ results,
+                                          uint32_t* __restrict__ snaps, uint8_t* __restrict__ out) {
+    const uint32_t lane = lid();
+    const BlockJob J = jobs[b];
+    const uint32_t np = results[b].npasses;
+    Tables t;
+    load_tables(t, tabs, J.band);
+    MqE m;
+    m.a = 0x8000u;
+    m.c = 0;
+    m.ct = 12;
+    m.pos = 0;
+    m.last = 0;
+    m.tab = t.mq;
+    // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
+    m.cx = (uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64);
+    if (lane == kPad || lane == kMarker) m.cx = 0;
+    m.w0 = 0;
+    m.w1 = 0;
+    m.wbase = 0;
+    m.out = out + J.out_off;
+    m.cap = J.out_cap;
+    m.overflow = 0;
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(pool + ((uint64_t)soff16[b] << 4));
+    const uint32_t* pe = pend + (size_t)b * kMaxPasses;
+    const uint32_t pe0 = lane < np ? pe[lane] >> 2 : 0u;  // pass ends in words
+    const uint32_t pe1 = lane + 64u < np && lane + 64u < kMaxPasses ? pe[lane + 64] >> 2 : 0u;
+    const uint32_t total_w = np <= 64u ? rl(pe0, np - 1u) : rl(pe1, np - 65u);
+    uint32_t k = 0, kend = rl(pe0, 0);
+    uint32_t snap0 = 0, snap1 = 0;
+    uint32_t nxt = lane < total_w ? words[lane] : 0u;
+    for (uint32_t base = 0; base < total_w; base += 64u) {
+        const uint32_t cur = nxt;
+        nxt = base + 64u + lane < total_w ? words[base + 64u + lane] : 0u;
+        const uint32_t cnt = min(64u, total_w - base);
+        for (uint32_t q = 0; q < cnt; q++) {
+            const uint32_t w = rl(cur, q);
+            mqe_symbol(m, w & 31u, w << 24);
+            mqe_symbol(m, (w >> 8) & 31u, w << 16);
+            mqe_symbol(m, (w >> 16) & 31u, w << 8);
+            mqe_symbol(m, (w >> 24) & 31u, w);
+            mqe_spill(m);
+            if (base + q + 1u == kend) {  // end of pass k
+                if (k < 64u) snap0 = wlane(snap0, m.pos, k);
+                else snap1 = wlane(snap1, m.pos, k - 64u);
+                k++;
+                kend = k < np ? (k < 64u ? rl(pe0, k) : rl(pe1, k - 64u)) : 0xffffffffu;
+            }
+        }
+    }
+    const uint32_t total = mqe_flush(m);
+    uint32_t* sp = snaps + (size_t)b * kMaxPasses;
+    if (lane < np) sp[lane] = snap0;
+    if (lane + 64u < np && lane + 64u < kMaxPasses) sp[lane + 64] = snap1;
+    if (lane == 0) {
+        results[b].nbytes = total;
+        results[b].overflow = (uint8_t)(m.overflow ? 1 : 0);
+    }
+}
+
+// Split the formed blocks: streams longer than frac x the longest go to the
+// scalar coder.  lists = {n_long, n_short, long[nb], short[nb]}; blocks with
+// nothing to code or whose stream did not fit are in neither.  One
+// workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void t1_split_kernel(int nb, const BlockResult* __restrict__ results,
+                                                        const uint32_t* __restrict__ pend, float frac,
+                                                        uint32_t* __restrict__ lists) {
+    __shared__ uint32_t smax, nlong, nshort;
+    if (threadIdx.x == 0) {
+        smax = 0;
+        nlong = 0;
+        nshort = 0;
+    }
+    __syncthreads();
+    uint32_t mx = 0;
+    for (int b = threadIdx.x; b < nb; b += 1024) {
+        const BlockResult r = results[b];
+        if (r.overflow == 0 && r.npasses) mx = max(mx, pend[(size_t)b * kMaxPasses + r.npasses - 1]);
+    }
+    atomicMax(&smax, mx);
+    __syncthreads();
+    const uint32_t thr = (uint32_t)(frac * (float)smax);
+    for (int b = threadIdx.x; b < nb; b += 1024) {
+        const BlockResult r = results[b];
+        if (r.overflow != 0 || r.npasses == 0) continue;
+        const uint32_t len = pend[(size_t)b * kMaxPasses + r.npasses - 1];
+        if (len > thr) lists[2 + atomicAdd(&nlong, 1u)] = (uint32_t)b;
+        else lists[2 + nb + atomicAdd(&nshort, 1u)] = (uint32_t)b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        lists[0] = nlong;
+        lists[1] = nshort;
+    }
+}
+
+// Workgroups [0, nb): the scalar coder on long stream i (if there is one);
+// then one lane-coder wave per 64 short streams.
+__global__ __launch_bounds__(64) void t1_hybrid_kernel(int nb, const BlockJob* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ tabs, const uint8_t* __restrict__ pool,
+                                                       const uint32_t* __restrict__ soff16,
+                                                       const uint32_t* __restrict__ pend, BlockResult* __restrict__ results,
+                                                       uint32_t* __restrict__ snaps, uint8_t* __restrict__ out,
+                                                       const uint32_t* __restrict__ lists) {
+    __shared__ uint32_t cst[32 * 64];
+    __shared__ uint32_t tab[64];
+    if ((int)blockIdx.x < nb) {
+        if (blockIdx.x >= lists[0]) return;
+        stream_mq((int)lists[2 + blockIdx.x], jobs, tabs, pool, soff16, pend, results, snaps, out);
+    } else {
+        const uint32_t g0 = (blockIdx.x - (uint32_t)nb) * 64u, nshort = lists[1];
+        if (g0 >= nshort) return;
+        const uint32_t j = g0 + lid();
+        lane_mq(j < nshort ? (int)lists[2 + nb + j] : -1, jobs, tabs, pool, soff16, pend, results, snaps, out, cst, tab);
+    }
 }
 
 // Tier-1 decoder: one wavefront per code-block.  Decodes `passes` coding
@@ -1394,6 +1542,13 @@ void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResu
 void launch_t1_mq(const BlockJob* jobs, int n, const uint32_t* tabs, const uint8_t* pool, const uint32_t* soff16,
                   const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, hipStream_t st) {
     if (n > 0) t1_mq_kernel<<<(n + 63) / 64, 64, 0, st>>>(n, jobs, tabs, pool, soff16, pend, res, snaps, out);
+}
+void launch_t1_hybrid(const BlockJob* jobs, int n, const uint32_t* tabs, const uint8_t* pool, const uint32_t* soff16,
+                      const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, uint32_t* lists, float frac,
+                      hipStream_t st) {
+    if (n <= 0) return;
+    t1_split_kernel<<<1, 1024, 0, st>>>(n, res, pend, frac, lists);
+    t1_hybrid_kernel<<<n + (n + 63) / 64, 64, 0, st>>>(n, jobs, tabs, pool, soff16, pend, res, snaps, out, lists);
 }
 uint64_t t1_stream_bound(uint32_t mbp) { return stream_bound(mbp); }
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st) {

@@ -344,6 +344,51 @@ def test_lane_coder_full_page(gpu, ora, lane_mode):
     assert j.encode_u8(px, 85) == ora.encode(px, 85)
 
 
+@pytest.fixture
+def hybrid_mode(lib):
+    lib.jp2g_set_t1_mode(4, 0)
+    yield lib
+    lib.jp2g_set_t1_mode(0, 8192)
+
+
+@pytest.mark.parametrize("name,make,qs", CASES, ids=[c[0] for c in CASES])
+def test_hybrid_coder_codestream_identical(gpu, ora, hybrid_mode, name, make, qs):
+    px = make()
+    for q in qs:
+        assert j.encode_u8(px, q) == ora.encode(px, q), f"{name} q{q}"
+    assert hybrid_mode.jp2g_last_t1_coder() == 4
+
+
+def test_hybrid_coder_block_records(gpu, ora, hybrid_mode):
+    # pass lengths (snapshots) from both halves of the split
+    px = synth.photo_page(300, 260, seed=9)
+    hybrid_mode.jp2g_set_debug(1)
+    try:
+        ours = j.encode_u8(px, 75)
+        data, enc = ora.encode(px, 75, keep=True)
+        assert ours == data
+        for i in range(hybrid_mode.jp2g_last_block_count()):
+            gi = np.zeros(16, np.int32)
+            hybrid_mode.jp2g_last_block_info(i, gi.ctypes.data)
+            assert gi.tolist() == enc.info(i).tolist(), i
+            npass = int(gi[11])
+            gp = np.zeros(5 * max(npass, 1), np.int32)
+            hybrid_mode.jp2g_last_block_passes(i, gp.ctypes.data)
+            assert np.array_equal(gp[:5 * npass].reshape(npass, 5), enc.passes(i)), i
+    finally:
+        hybrid_mode.jp2g_set_debug(0)
+
+
+def test_hybrid_coder_mixed_batch(gpu, ora, hybrid_mode):
+    pages = list(synth.mixed_pages(6, seed=7, scale=0.15))
+    assert j.encode_many(pages) == [ora.encode(px, q) for px, q in pages]
+
+
+def test_hybrid_coder_photo_page(gpu, ora, hybrid_mode):
+    px = synth.photo_page(4000, 3000, seed=3)
+    assert j.encode_u8(px, 75) == ora.encode(px, 75)
+
+
 def test_threshold_mode_batch_crosses_threshold(gpu, ora, lib):
     """With a block threshold, a batch above it runs the lane coder, one
     below it the wave coder; both give the single-image bytes."""
@@ -361,17 +406,20 @@ def test_threshold_mode_batch_crosses_threshold(gpu, ora, lib):
 
 
 def test_tuned_mode_probes_then_reuses(gpu, ora, lib):
-    """Tuned mode: the first batch of a size class runs both coders (output
-    unchanged), later batches of the class reuse the choice."""
+    """Tuned mode: the first batch of a size class runs the three coders
+    (output unchanged), later batches of the class reuse the choice."""
     pages = list(synth.mixed_pages(5, scale=0.12))
     singles = [ora.encode(px, q) for px, q in pages]
     lib.jp2g_set_t1_mode(0, 50)
     try:
         first = j.encode_many(pages)
-        chosen = lib.jp2g_last_t1_lanes()
         assert first == singles
+        assert lib.jp2g_last_t1_coder() in (1, 2)  # the probe's own run: lanes, or the wave coder
         for _ in range(2):
             assert j.encode_many(pages) == singles
-            assert lib.jp2g_last_t1_lanes() == 0 or chosen == 1
+            chosen = lib.jp2g_last_t1_coder()
+            assert chosen in (1, 2, 4)
+        assert j.encode_many(pages) == singles
+        assert lib.jp2g_last_t1_coder() == chosen
     finally:
         lib.jp2g_set_t1_mode(0, 8192)
