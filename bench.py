@@ -374,6 +374,68 @@ def main():
                 "batched_equals_single": bool(same), "parity_vs_oracle_smallest_page": bool(ok)}
             del devp, outs
 
+            # mixed serving: Poisson arrivals of 60 % encodes of the queue's
+            # pages and 40 % decodes of their codestreams, host memory; each
+            # result is checked against the same request run alone
+            if args.serve_seconds > 0:
+                import threading
+
+                want_enc = {}
+                want_dec = {k: j.decode_u8(streams[k]) for k in dec_ids}
+                nreq = max(1, int(args.serve_rate * args.serve_seconds))
+                rng = np.random.default_rng(55)
+                arr = np.cumsum(rng.exponential(1.0 / args.serve_rate, nreq))
+                is_enc = rng.uniform(size=nreq) < 0.6
+                pick = [int(rng.integers(0, len(host))) if e else dec_ids[int(rng.integers(0, len(dec_ids)))]
+                        for e in is_enc]
+                for k in set(p for p, e in zip(pick, is_enc) if e):
+                    want_enc[k] = j.encode_u8(host[k][0], host[k][1])
+                with j.Server(slots=2, mem_budget_mib=32768, max_batch=16) as srv:
+                    def stream_once():
+                        rids = [None] * nreq
+                        rdone = [None] * nreq
+
+                        def waiter2():
+                            for i in range(nreq):
+                                while rids[i] is None:
+                                    time.sleep(0.0002)
+                                rdone[i] = srv.wait(rids[i])
+
+                        th = threading.Thread(target=waiter2)
+                        th.start()
+                        t0m = time.perf_counter()
+                        for i in range(nreq):
+                            dly = t0m + arr[i] - time.perf_counter()
+                            if dly > 0:
+                                time.sleep(dly)
+                            k = pick[i]
+                            rids[i] = (srv.submit_encode(host[k][0], host[k][1]) if is_enc[i]
+                                       else srv.submit_decode(streams[k]))
+                        th.join()
+                        return rdone, t0m, time.perf_counter()
+
+                    # the first pass warms the arenas, the pinned pool and the
+                    # coder tuner's size classes; the second is reported
+                    stream_once()
+                    rdone, t0m, t_endm = stream_once()
+                okm = all((rdone[i].value == want_enc[pick[i]]) if is_enc[i]
+                          else np.array_equal(rdone[i].value, want_dec[pick[i]]) for i in range(nreq))
+                lat = np.array([r.total_ms for r in rdone])
+                le = lat[is_enc]
+                ld = lat[~is_enc]
+                served_mpx = sum(host[pick[i]][0].shape[0] * host[pick[i]][0].shape[1] for i in range(nreq)) / 1e6
+                configs["mixed_serving"] = {
+                    "workload": "Poisson arrivals: 60 % encodes of the mixed-queue pages (host pixels), "
+                                "40 % decodes of their codestreams", "rate_req_s": args.serve_rate,
+                    "requests": nreq, "encodes": int(is_enc.sum()), "slots": 2,
+                    "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                    "encode_p99_ms": round(float(np.percentile(le, 99)), 2) if le.size else None,
+                    "decode_p99_ms": round(float(np.percentile(ld, 99)), 2) if ld.size else None,
+                    "max_ms": round(float(lat.max()), 2), "throughput_req_s": round(nreq / (t_endm - t0m), 1),
+                    "served_mpx_s": round(served_mpx / (t_endm - t0m), 1),
+                    "mean_batch": round(float(np.mean([r.batch_size for r in rdone])), 2),
+                    "results_equal_alone": bool(okm), "note": "second pass of the same arrivals (steady state)"}
+
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib << 20
         a = torch.ones(n, dtype=torch.uint8, device=dev)

@@ -527,13 +527,17 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                              (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), (uint32_t*)(D + o_lists), split, st);
     };
     int run = probe ? 2 : coder;
+    // From 4 x lane_min_blocks up the wave coder has never been the fastest
+    // (it is the slowest by far there), so the probe skips it.
+    const bool probe_wave = probe && (uint64_t)nb < 4ull * lane_min_blocks;
     if (probe) {
         // the wave coder and the hybrid at both split points are timed
         // first; the lane coder then runs below and its (identical) results
         // are the ones used
         J2L_HIP(hipEventRecord(ev.e[4], st));
-        launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
-                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
+        if (probe_wave)
+            launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots),
+                             (BlockResult*)(D + o_res), (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
         J2L_HIP(hipEventRecord(ev.e[5], st));
         run_streams(4, kProbeFrac[0]);
         J2L_HIP(hipEventRecord(ev.e[6], st));
@@ -597,7 +601,8 @@ t1_again:
         }
     }
     if (probe) {
-        const float t_wave = ev.ms(4, 5), t_h0 = ev.ms(5, 6), t_h1 = ev.ms(6, 7), t_lane = ev.ms(1, 2);
+        const float t_wave = probe_wave ? ev.ms(4, 5) : 1e30f, t_h0 = ev.ms(5, 6), t_h1 = ev.ms(6, 7),
+                    t_lane = ev.ms(1, 2);
         T1Choice best{1, hybrid_frac};
         if (run != 1) {  // the streams fitted the pool
             float t = t_wave;
