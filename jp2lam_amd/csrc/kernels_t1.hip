@@ -338,53 +338,6 @@ __device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t b
     return newsig;
 }
 
-// Magnitude refinement pass (t1.rs:914-965, decode/t1.rs:268-315): members
-// and contexts are independent of the pass's own decisions, so they are
-// formed lane-parallel; only the coding is serial.  Per stripe-column the
-// word holds the member rows (bits 0..3; rows past the block are never
-// members) and each row's context index (5 bits at 4 + 5 ci), so the row
-// loop unrolls and a decision's context is one bit-field extract.
-template <bool DEC>
-__device__ void mr_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t& plane, int w, int h) {
-    const int lane = lid();
-    const int nstripes = (h + 3) >> 2;
-    for (int s = 0; s < nstripes; s++) {
-        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
-        const uint32_t rmask = (1u << nrows) - 1u;
-        uint32_t sig6 = win6(st.sig, s);
-        uint32_t sig4 = (sig6 >> 1) & 15u, vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
-        uint32_t ref4 = (uint32_t)(st.ref >> (4 * s)) & 15u;
-        uint32_t mem = sig4 & ~vis4 & rmask;
-        uint32_t w6 = __shfl_up(sig6, 1, 64), e6 = __shfl_down(sig6, 1, 64);
-        if (lane == 0) w6 = 0;
-        if (lane == 63) e6 = 0;
-        uint32_t Q = mem;
-#pragma unroll
-        for (int ci = 0; ci < 4; ci++) {
-            const uint32_t r = (uint32_t)ci + 1;
-            const uint32_t any = nbmask(w6, sig6, e6, r) != 0;
-            const uint32_t ctx = ((ref4 >> ci) & 1u) ? CTX_MAG + 2u : CTX_MAG + any;
-            Q |= ctx << (4 + 5 * ci);
-        }
-        uint64_t M = ballot(lane < w && mem != 0);
-        uint32_t upd = 0;
-        while (M) {
-            int x = __builtin_ctzll(M);
-            M &= M - 1;
-            cd.column();
-            const uint32_t q = rl(Q, x);
-            uint32_t ones = 0;
-#pragma unroll
-            for (int ci = 0; ci < 4; ci++)
-                if ((q >> ci) & 1u) ones |= cd.code(t, (q >> (4 + 5 * ci)) & 31u, 0) << ci;
-            upd = wl(upd, ones, x);
-        }
-        plane |= (uint64_t)(upd & 15u) << (4 * s);
-    }
-    (void)bits;
-    st.ref |= st.sig & ~st.vis;
-}
-
 // Cleanup pass with run-length mode on full clean stripes (t1.rs:977-1197,
 // decode/t1.rs:317-446).
 template <bool DEC>
@@ -1469,12 +1422,121 @@ __global__ __launch_bounds__(64) void t1_hybrid_kernel(int nb, const BlockJob* _
     }
 }
 
+// Magnitude refinement pass of the decoder (t1.rs:914-965, decode/t1.rs:268-315).
+// Members and contexts do not depend on the pass's own decisions, so the
+// pass is formed lane-parallel into LDS in scan order (as the encoder forms
+// it), decoded by one scalar loop that appends each decision to a bit
+// string (one bit-field extract into M0 and one shift-add per decision),
+// and the bits go back to their columns lane-parallel.  symw: 1024 words
+// (at most 4096 members), bitw: 128 words.
+__device__ __forceinline__ void mr_pass_dec(Coder<true>& cd, const Tables& t, Col& st, uint64_t& plane, int w, int h,
+                                            uint32_t* symw, uint32_t* bitw) {
+    const uint32_t lane = lid();
+    const bool valid = (int)lane < w;
+    const int nstripes = (h + 3) >> 2;
+    uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
+    auto members = [&](int s, uint32_t& sig6, uint32_t& ref4) {
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
+        const uint32_t rmask = valid ? (1u << nrows) - 1u : 0u;
+        sig6 = win6(st.sig, s);
+        ref4 = (uint32_t)(st.ref >> (4 * s)) & 15u;
+        const uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
+        return (sig6 >> 1) & ~vis4 & rmask;
+    };
+    uint32_t total = 0;
+    for (int s = 0; s < nstripes; s++) {
+        uint32_t sig6, ref4;
+        const uint32_t mem = members(s, sig6, ref4);
+        const uint32_t w6 = from_west(sig6), e6 = from_east(sig6);
+        uint32_t tot;
+        uint32_t pos = total + lane_offsets((uint32_t)__builtin_popcount(mem), tot);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if ((mem >> r) & 1u) {
+                const uint32_t ctx = ((ref4 >> r) & 1u) ? CTX_MAG + 2u : (nbmask(w6, sig6, e6, r + 1) ? CTX_MAG + 1u : CTX_MAG);
+                symb[pos++] = (uint8_t)ctx;
+            }
+        }
+        total += tot;
+    }
+    __syncthreads();
+    // the scan slides the segment window at least every 8 decisions (at
+    // most 16 bytes read)
+    uint32_t acc = 0;
+    for (uint32_t base = 0; base < total; base += 256u) {
+        const uint32_t sv = symw[(base >> 2) + lane];
+        const uint32_t cnt = min(256u, total - base);
+        uint32_t q = 0;
+        for (; q + 8u <= cnt; q += 8u) {
+            cd.column();
+            const uint32_t wd = rl(sv, q >> 2), we = rl(sv, (q >> 2) + 1u);
+            acc = acc * 2u + cd.code(t, wd & 31u, 0);
+            acc = acc * 2u + cd.code(t, (wd >> 8) & 31u, 0);
+            acc = acc * 2u + cd.code(t, (wd >> 16) & 31u, 0);
+            acc = acc * 2u + cd.code(t, (wd >> 24) & 31u, 0);
+            acc = acc * 2u + cd.code(t, we & 31u, 0);
+            acc = acc * 2u + cd.code(t, (we >> 8) & 31u, 0);
+            acc = acc * 2u + cd.code(t, (we >> 16) & 31u, 0);
+            acc = acc * 2u + cd.code(t, (we >> 24) & 31u, 0);
+            const uint32_t done = base + q + 8u;
+            if ((done & 31u) == 0u) {
+                if (lane == 0) bitw[(done >> 5) - 1u] = acc;
+                acc = 0;
+            }
+        }
+        for (; q + 4u <= cnt; q += 4u) {
+            cd.column();
+            const uint32_t wd = rl(sv, q >> 2);
+            acc = acc * 2u + cd.code(t, wd & 31u, 0);
+            acc = acc * 2u + cd.code(t, (wd >> 8) & 31u, 0);
+            acc = acc * 2u + cd.code(t, (wd >> 16) & 31u, 0);
+            acc = acc * 2u + cd.code(t, (wd >> 24) & 31u, 0);
+            const uint32_t done = base + q + 4u;
+            if ((done & 31u) == 0u) {
+                if (lane == 0) bitw[(done >> 5) - 1u] = acc;
+                acc = 0;
+            }
+        }
+        if (q < cnt) {
+            cd.column();
+            uint32_t wd = rl(sv, q >> 2);
+            for (; q < cnt; q++, wd >>= 8) acc = acc * 2u + cd.code(t, wd & 31u, 0);
+        }
+    }
+    if (total & 31u) {  // the last, partial word, first decision in bit 31
+        acc <<= 32u - (total & 31u);
+        if (lane == 0) bitw[total >> 5] = acc;
+    }
+    __syncthreads();
+    total = 0;
+    for (int s = 0; s < nstripes; s++) {
+        uint32_t sig6, ref4;
+        const uint32_t mem = members(s, sig6, ref4);
+        uint32_t tot;
+        uint32_t pos = total + lane_offsets((uint32_t)__builtin_popcount(mem), tot);
+        uint32_t ones = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if ((mem >> r) & 1u) {
+                ones |= ((bitw[pos >> 5] >> (31u - (pos & 31u))) & 1u) << r;
+                pos++;
+            }
+        }
+        plane |= (uint64_t)ones << (4 * s);
+        total += tot;
+    }
+    st.ref |= st.sig & ~st.vis;
+    __syncthreads();  // symw and bitw are reused by the next pass
+}
+
 // Tier-1 decoder: one wavefront per code-block.  Decodes `passes` coding
 // passes and writes the reconstructed coefficients into the plane:
 // dequantised f32 (sign * (|q| + 0.5)) * step for 9/7 (scalar.rs:233-240),
 // i32 for 5/3.  Blocks that are not included are written as zeros.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap))) void t1_decode_kernel(const DecodeJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                        const uint8_t* __restrict__ payload) {
+    __shared__ uint32_t symw[1024];
+    __shared__ uint32_t bitw[128];
     const int lane = lid();
     const DecodeJob J = jobs[blockIdx.x];
     const int w = __builtin_amdgcn_readfirstlane(J.w), h = __builtin_amdgcn_readfirstlane(J.h);
@@ -1502,7 +1564,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
             if (p != mbp - 1) {
                 sp_pass<true>(cd, t, st, 0, 0, plane, w, h);
                 if (--remaining == 0) goto plane_done;
-                mr_pass<true>(cd, t, st, 0, plane, w, h);
+                mr_pass_dec(cd, t, st, plane, w, h, symw, bitw);
                 if (--remaining == 0) goto plane_done;
             }
             cl_pass<true>(cd, t, st, 0, 0, plane, w, h);
