@@ -802,6 +802,20 @@ __device__ __forceinline__ void mqe_pass(MqE& m, const uint32_t* symw, uint32_t 
         const uint32_t sv = symw[(base >> 2) + lid()];
         const uint32_t cnt = min(256u, n - base);
         uint32_t q = 0;
+        // 8 symbols move at most 16 bytes out, so one spill check per 8
+        // keeps the 128-byte window from filling
+        for (; q + 8u <= cnt; q += 8u) {
+            const uint32_t w = rl(sv, q >> 2), x = rl(sv, (q >> 2) + 1u);
+            mqe_symbol(m, w & 31u, w << 24);
+            mqe_symbol(m, (w >> 8) & 31u, w << 16);
+            mqe_symbol(m, (w >> 16) & 31u, w << 8);
+            mqe_symbol(m, (w >> 24) & 31u, w);
+            mqe_symbol(m, x & 31u, x << 24);
+            mqe_symbol(m, (x >> 8) & 31u, x << 16);
+            mqe_symbol(m, (x >> 16) & 31u, x << 8);
+            mqe_symbol(m, (x >> 24) & 31u, x);
+            mqe_spill(m);
+        }
         for (; q + 4u <= cnt; q += 4u) {
             const uint32_t w = rl(sv, q >> 2);
             mqe_symbol(m, w & 31u, w << 24);
