@@ -84,8 +84,9 @@ __device__ __forceinline__ uint32_t sc_entry(const Tables& t, uint32_t si) { ret
 // MPS in bit 31 (as in the encoder), kept in s[80 + k].  The segment is read through a
 // 128-byte window of two VGPRs (lane i of w0/w1 = byte dbase + i / + 64 + i,
 // 0xFF past the end, which the decoder reads as the reference does,
-// mq/mod.rs:152-154); the scan slides it before each column, which reads
-// fewer than 24 bytes, so a decision never leaves it.  The next 64 bytes
+// mq/mod.rs:152-154); the scan slides it before each column or group of at
+// most 16 decisions, which read at most 32 bytes (two byte-ins per decision),
+// and a slide leaves at least 40 bytes ahead, so a decision never leaves it.  The next 64 bytes
 // (w2) are loaded one slide ahead, so a slide never waits on memory.
 struct MqDec {
     uint32_t a, c, ct, bp, cur;  // cur = byte at bp
@@ -105,7 +106,7 @@ __device__ __forceinline__ void dec_window_init(MqDec& d) {
     d.w2 = dec_load64(d, 128);
 }
 __device__ __forceinline__ void dec_slide(MqDec& d) {
-    if (d.bp + 24u >= d.dbase + 128u) {
+    if (d.bp + 40u >= d.dbase + 128u) {
         d.w0 = d.w1;
         d.w1 = d.w2;
         d.dbase += 64u;
@@ -802,18 +803,17 @@ __device__ __forceinline__ void mqe_pass(MqE& m, const uint32_t* symw, uint32_t 
         const uint32_t sv = symw[(base >> 2) + lid()];
         const uint32_t cnt = min(256u, n - base);
         uint32_t q = 0;
-        // 8 symbols move at most 16 bytes out, so one spill check per 8
-        // keeps the 128-byte window from filling
-        for (; q + 8u <= cnt; q += 8u) {
-            const uint32_t w = rl(sv, q >> 2), x = rl(sv, (q >> 2) + 1u);
-            mqe_symbol(m, w & 31u, w << 24);
-            mqe_symbol(m, (w >> 8) & 31u, w << 16);
-            mqe_symbol(m, (w >> 16) & 31u, w << 8);
-            mqe_symbol(m, (w >> 24) & 31u, w);
-            mqe_symbol(m, x & 31u, x << 24);
-            mqe_symbol(m, (x >> 8) & 31u, x << 16);
-            mqe_symbol(m, (x >> 16) & 31u, x << 8);
-            mqe_symbol(m, (x >> 24) & 31u, x);
+        // 16 symbols move at most 32 bytes out, so one spill check per 16
+        // keeps the 128-byte window from filling (at most 96 bytes pending)
+        for (; q + 16u <= cnt; q += 16u) {
+#pragma unroll
+            for (uint32_t g = 0; g < 4u; g++) {
+                const uint32_t w = rl(sv, (q >> 2) + g);
+                mqe_symbol(m, w & 31u, w << 24);
+                mqe_symbol(m, (w >> 8) & 31u, w << 16);
+                mqe_symbol(m, (w >> 16) & 31u, w << 8);
+                mqe_symbol(m, (w >> 24) & 31u, w);
+            }
             mqe_spill(m);
         }
         for (; q + 4u <= cnt; q += 4u) {
@@ -1474,25 +1474,23 @@ __device__ __forceinline__ void mr_pass_dec(Coder<true>& cd, const Tables& t, Co
         total += tot;
     }
     __syncthreads();
-    // the scan slides the segment window at least every 8 decisions (at
-    // most 16 bytes read)
+    // the scan slides the segment window at least every 16 decisions
     uint32_t acc = 0;
     for (uint32_t base = 0; base < total; base += 256u) {
         const uint32_t sv = symw[(base >> 2) + lane];
         const uint32_t cnt = min(256u, total - base);
         uint32_t q = 0;
-        for (; q + 8u <= cnt; q += 8u) {
+        for (; q + 16u <= cnt; q += 16u) {
             cd.column();
-            const uint32_t wd = rl(sv, q >> 2), we = rl(sv, (q >> 2) + 1u);
-            acc = acc * 2u + cd.code(t, wd & 31u, 0);
-            acc = acc * 2u + cd.code(t, (wd >> 8) & 31u, 0);
-            acc = acc * 2u + cd.code(t, (wd >> 16) & 31u, 0);
-            acc = acc * 2u + cd.code(t, (wd >> 24) & 31u, 0);
-            acc = acc * 2u + cd.code(t, we & 31u, 0);
-            acc = acc * 2u + cd.code(t, (we >> 8) & 31u, 0);
-            acc = acc * 2u + cd.code(t, (we >> 16) & 31u, 0);
-            acc = acc * 2u + cd.code(t, (we >> 24) & 31u, 0);
-            const uint32_t done = base + q + 8u;
+#pragma unroll
+            for (uint32_t g = 0; g < 4u; g++) {
+                const uint32_t wd = rl(sv, (q >> 2) + g);
+                acc = acc * 2u + cd.code(t, wd & 31u, 0);
+                acc = acc * 2u + cd.code(t, (wd >> 8) & 31u, 0);
+                acc = acc * 2u + cd.code(t, (wd >> 16) & 31u, 0);
+                acc = acc * 2u + cd.code(t, (wd >> 24) & 31u, 0);
+            }
+            const uint32_t done = base + q + 16u;
             if ((done & 31u) == 0u) {
                 if (lane == 0) bitw[(done >> 5) - 1u] = acc;
                 acc = 0;
