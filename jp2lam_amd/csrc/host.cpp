@@ -226,10 +226,22 @@ int zc_index_context(int orient, int idx) {
 }
 }  // namespace
 
+// Words 0..63: the MQ states as the coders step them.  Word i < 47 is state
+// i: qe | nmps << 16 | nlps' << 22.  The LPS targets of the three switching
+// states (0 -> 1, 6 -> 6, 14 -> 14) get copies at words 47..49 with bit 31
+// set, and nlps' points there, so the next state of any decision is
+// word[field] XOR (MPS in bit 31): the copy's bit 31 flips the MPS.
 std::array<uint32_t, 384> make_t1_tables() {
     std::array<uint32_t, 384> t{};
-    for (int i = 0; i < 47; i++)
-        t[i] = kQe[i] | ((uint32_t)kNmps[i] << 16) | ((uint32_t)kNlps[i] << 22) | ((uint32_t)kSwitch(i) << 28);
+    auto word = [](int i) {
+        const int l = kNlps[i];
+        const uint32_t nl = kSwitch(i) ? (l == 1 ? 47u : l == 6 ? 48u : 49u) : (uint32_t)l;
+        return kQe[i] | ((uint32_t)kNmps[i] << 16) | (nl << 22);
+    };
+    for (int i = 0; i < 47; i++) t[i] = word(i);
+    t[47] = word(1) | 0x80000000u;
+    t[48] = word(6) | 0x80000000u;
+    t[49] = word(14) | 0x80000000u;
     for (int o = 0; o < 4; o++)
         for (int idx = 0; idx < 512; idx++) t[64 + o * 64 + idx / 8] |= (uint32_t)zc_index_context(o, idx) << ((idx % 8) * 4);
     for (int i = 0; i < 256; i++) {

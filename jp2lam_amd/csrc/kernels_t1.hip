@@ -70,7 +70,7 @@ __device__ __forceinline__ uint32_t win6(uint64_t m, int s) {
 }
 
 struct Tables {
-    uint32_t mq;  // lane i < 47: qe | nmps << 16 | nlps << 22 | switch << 28
+    uint32_t mq;  // lane i < 50: qe | nmps << 16 | nlps << 22 | flip-MPS << 31
     uint32_t zc;  // lane i: ZC contexts of neighbour masks 8i..8i+7 (nibbles)
     uint32_t sc;  // lane i: sign entries 4i..4i+3 (ctx | spb << 4, bytes)
 };
@@ -161,8 +161,8 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_lshr_b32 %[bit], %[e], 31\n\t"   // MPS, a stays >= 0x8000
         "s_branch 9f\n"
         "1:\n\t"                     // lower sub-interval: a = qe; LPS iff a >= qe
-        "s_cmp_lt_u32 %[a], %[q]\n\t"
-        "s_cselect_b32 %[k], 0, -1\n\t"
+        "s_cmp_ge_u32 %[a], %[q]\n\t"
+        "s_cselect_b32 %[k], -1, 0\n\t"
         "s_mov_b32 %[a], %[q]\n\t"
         "s_branch 8f\n"
         "2:\n\t"                        // shift with byte-in whenever ct is 0
@@ -208,18 +208,14 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "7:\n\t"                        // upper sub-interval, a < 0x8000: LPS iff a < qe
         "s_cmp_lt_u32 %[a], %[q]\n\t"
         "s_cselect_b32 %[k], -1, 0\n"
-        "8:\n\t"                    // k = all ones on LPS
-        "s_xor_b32 %[bit], %[e], %[k]\n\t"   // bit 31: MPS ^ LPS = the decision
-        "s_lshr_b32 %[bit], %[bit], 31\n\t"
-        "s_lshl_b32 %[n0], %[e], 3\n\t"
-        "s_and_b32 %[n0], %[n0], %[k]\n\t"
-        "s_xor_b32 %[n0], %[n0], %[e]\n\t"
-        "s_and_b32 %[n0], %[n0], 0x80000000\n\t"
-        "s_cmp_lg_u32 %[k], 0\n\t"
+        "8:\n\t"                    // k = all ones on LPS, and so is SCC
         "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
         "s_bfe_u32 %[t], %[e], %[t]\n\t"
         "v_readlane_b32 %[t], %[tab], %[t]\n\t"
-        "s_or_b32 %[t], %[t], %[n0]\n\t"
+        "s_xor_b32 %[bit], %[e], %[k]\n\t"   // bit 31: MPS ^ LPS = the decision
+        "s_lshr_b32 %[bit], %[bit], 31\n\t"
+        "s_and_b32 %[n0], %[e], 0x80000000\n\t"
+        "s_xor_b32 %[t], %[t], %[n0]\n\t"   // bit 31: MPS, flipped by a switching LPS
         "s_movreld_b32 s80, %[t]\n\t"
         "s_flbit_i32_b32 %[t], %[a]\n\t"
         "s_sub_u32 %[t], %[t], 16\n\t"
@@ -681,8 +677,10 @@ __device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t 
 
 // ---- MQ encoder (mq/mod.rs:163-432) --------------------------------------------
 // Coder registers in SGPRs.  cx lane k holds context k's current state as
-// its table entry (qe | nmps << 16 | nlps << 22 | switch << 28) with the
-// MPS in bit 31, so a decision needs one readlane.  (Keeping the states in
+// its table entry (qe | nmps << 16 | nlps << 22) with the MPS in bit 31, so
+// a decision needs one readlane; the next state is the table entry at the
+// NMPS or NLPS field XOR the MPS bit (entries 47..49, the LPS targets of the
+// switching states, carry bit 31 to flip it).  (Keeping the states in
 // an SGPR block as the decoder does measured no faster here: the state
 // write sits early enough in the renormalising path to hide its latency.)  Output bytes collect in
 // a 128-byte window of two VGPRs (lane i of w0/w1 = byte wbase + i / + 64 + i);
@@ -766,11 +764,8 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
         "s_bfe_u32 %[t], %[e], %[t]\n\t"
         "v_readlane_b32 %[t], %[tab], %[t]\n\t"
-        "s_lshl_b32 %[q], %[e], 3\n\t"               // bit 31: SWITCH
-        "s_and_b32 %[q], %[q], %[u]\n\t"             // SWITCH on LPS only
-        "s_xor_b32 %[q], %[q], %[e]\n\t"             // bit 31: the new MPS
-        "s_and_b32 %[q], %[q], 0x80000000\n\t"
-        "s_or_b32 %[t], %[t], %[q]\n\t"
+        "s_and_b32 %[q], %[e], 0x80000000\n\t"
+        "s_xor_b32 %[t], %[t], %[q]\n\t"             // bit 31: MPS, flipped by a switching LPS
         "v_writelane_b32 %[cx], %[t], m0\n\t"
         "s_flbit_i32_b32 %[t], %[a]\n\t"
         "s_sub_u32 %[t], %[t], 16\n"
@@ -1244,8 +1239,7 @@ __device__ __forceinline__ void lane_mq(int b, const BlockJob* __restrict__ jobs
         m.a = to_qe ? qe : a2;
         m.c += to_qe ? 0u : qe;
         const uint32_t next = (e >> (lps ? 22 : 16)) & 63u;
-        const uint32_t mps = (lps ? e ^ (e << 3) : e) & 0x80000000u;
-        const uint32_t ne = tab[next] | mps;
+        const uint32_t ne = tab[next] ^ (e & 0x80000000u);
         *cp = keep ? e : ne;
         uint32_t sh = (uint32_t)__builtin_clz(m.a) - 16u;
         while (sh >= m.ct) {
