@@ -615,6 +615,7 @@ jp2lam_status jp2g_t1_encode_block(const int32_t* coeffs, uint32_t w, uint32_t h
         uint32_t* dsnap;
         uint16_t* dns;
         uint32_t* dtab;
+        uint32_t* dsched;
         J2L_HIP(hipMalloc((void**)&dplane, 4 * w * h));
         J2L_HIP(hipMalloc((void**)&dout, slot));
         J2L_HIP(hipMalloc((void**)&djob, sizeof(BlockJob)));
@@ -622,6 +623,7 @@ jp2lam_status jp2g_t1_encode_block(const int32_t* coeffs, uint32_t w, uint32_t h
         J2L_HIP(hipMalloc((void**)&dsnap, 4 * kMaxPasses));
         J2L_HIP(hipMalloc((void**)&dns, 2 * kMaxPasses));
         J2L_HIP(hipMalloc((void**)&dtab, 4 * tabs.size()));
+        J2L_HIP(hipMalloc((void**)&dsched, 4 * t1_sched_words(1)));
         BlockJob j{};
         j.plane = (uint64_t)(uintptr_t)dplane;
         j.stride = w;
@@ -635,7 +637,7 @@ jp2lam_status jp2g_t1_encode_block(const int32_t* coeffs, uint32_t w, uint32_t h
         J2L_HIP(hipMemcpyAsync(dplane, coeffs, 4 * w * h, hipMemcpyHostToDevice, st));
         J2L_HIP(hipMemcpyAsync(djob, &j, sizeof(j), hipMemcpyHostToDevice, st));
         J2L_HIP(hipMemcpyAsync(dtab, tabs.data(), 4 * tabs.size(), hipMemcpyHostToDevice, st));
-        launch_t1_encode(djob, 1, dtab, dout, dres, dsnap, dns, st);
+        launch_t1_encode(djob, 1, dtab, dout, dres, dsnap, dns, dsched, st);
         BlockResult r;
         uint32_t snaps[kMaxPasses];
         uint16_t ns[kMaxPasses];
@@ -652,6 +654,7 @@ jp2lam_status jp2g_t1_encode_block(const int32_t* coeffs, uint32_t w, uint32_t h
         (void)hipFree(dsnap);
         (void)hipFree(dns);
         (void)hipFree(dtab);
+        (void)hipFree(dsched);
         if (r.overflow) return fail(ENCODE_FAILED, "code-block segment exceeded its device slot");
         info[0] = r.mbp;
         info[1] = r.zbp;

@@ -149,8 +149,11 @@ void launch_fwd_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles,
 void launch_inv_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles, hipStream_t st);
 void launch_prepare(const PixSeg* segs, int nseg, uint32_t nblocks, hipStream_t st);
 void launch_finalize(const PixSeg* segs, int nseg, uint32_t nblocks, hipStream_t st);
+// The wave coder: a cost estimate and a sort of the blocks, then one wave per
+// block in that order.  sched: t1_sched_words(n) words of device scratch.
+size_t t1_sched_words(int n);
 void launch_t1_encode(const BlockJob* jobs, int n, const uint32_t* tabs, uint8_t* out, BlockResult* res, uint32_t* snaps,
-                      uint16_t* newsigs, hipStream_t st);
+                      uint16_t* newsigs, uint32_t* sched, hipStream_t st);
 void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResult* res, uint32_t* pend, uint16_t* newsigs,
                     uint32_t* soff16, uint8_t* pool, unsigned long long* pool_used, unsigned long long pool_cap,
                     hipStream_t st);

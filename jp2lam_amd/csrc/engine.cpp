@@ -269,6 +269,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     const size_t o_blkimg = L.add(4 * nb);
     const size_t o_packbase = L.add(8 * imgs.size());
     const size_t o_packed = L.add(pack_total);
+    const size_t o_sched = L.add(4 * t1_sched_words(nb));
     // Lane-per-block Tier-1 (large batches): symbol streams, bounded per
     // block by the band's Mb + 2 bit-planes; an overflow falls back to the
     // wave-per-block kernel below.
@@ -537,7 +538,8 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         J2L_HIP(hipEventRecord(ev.e[4], st));
         if (probe_wave)
             launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots),
-                             (BlockResult*)(D + o_res), (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
+                             (BlockResult*)(D + o_res), (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns),
+                             (uint32_t*)(D + o_sched), st);
         J2L_HIP(hipEventRecord(ev.e[5], st));
         run_streams(4, kProbeFrac[0]);
         J2L_HIP(hipEventRecord(ev.e[6], st));
@@ -550,7 +552,7 @@ t1_again:
         run_streams(run, frac);
     } else {
         launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
-                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), st);
+                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), (uint32_t*)(D + o_sched), st);
     }
     J2L_HIP(hipEventRecord(ev.e[2], st));
     // PCRD: lossless images keep every pass; lossy ones go through the hull.

@@ -683,15 +683,49 @@ __device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t 
 // switching states, carry bit 31 to flip it).  (Keeping the states in
 // an SGPR block as the decoder does measured no faster here: the state
 // write sits early enough in the renormalising path to hide its latency.)  Output bytes collect in
-// a 128-byte window of two VGPRs (lane i of w0/w1 = byte wbase + i / + 64 + i);
-// the lower 64 are stored with one coalesced write once final.
+// a 64-byte ring in one VGPR: byte b sits in lane (b + 1) & 63, so the byte
+// committed when pos moves on goes to lane pos & 63 with no range test (the
+// first commit, of no byte, lands in lane 0 as "byte -1", never stored).
+// Bytes [wbase, wbase + 32) are stored with one write once final; between
+// spill checks at most 32 bytes are added, so no unstored byte is reused.
 struct MqE {
     uint32_t a, c, ct, pos, last;
     uint32_t cx, tab;       // VGPRs
-    uint32_t w0, w1, wbase; // byte window
+    uint32_t win, wbase;    // byte ring (VGPR) and the first unstored byte
     uint8_t* out;
     uint32_t cap, overflow;
 };
+
+// Byte-out (mq/mod.rs:373-414) after the shift by ct: the usual case (held
+// byte not 0xFF, no carry) runs straight to the commit; then back to the
+// renormalisation test (BACK: label reference with its direction).  The
+// commit writes the held byte to ring lane pos & 63 (M0 is free here: the
+// state write is done).  L8, L7, LCOM, LCARRY are label numbers local to the
+// caller: 8-bit byte, 7-bit byte, commit, carry into the held byte.
+#define MQE_BYTEOUT(L8, L7, LCOM, LCARRY, BACK)                                    \
+    "s_lshl_b32 %[a], %[a], %[ct]\n\t"                                          \
+    "s_lshl_b32 %[c], %[c], %[ct]\n\t"                                          \
+    "s_sub_u32 %[t], %[t], %[ct]\n\t"                                           \
+    "s_cmp_eq_u32 %[last], 0xff\n\t"                                            \
+    "s_cbranch_scc1 " L7 "f\n\t"                                                \
+    "s_bitcmp1_b32 %[c], 27\n\t"                                                \
+    "s_cbranch_scc1 " LCARRY "f\n" L8 ":\n\t"                                  \
+    "s_lshr_b32 %[q], %[c], 19\n\t"                                             \
+    "s_and_b32 %[c], %[c], 0x7ffff\n\t"                                         \
+    "s_mov_b32 %[ct], 8\n" LCOM ":\n\t"                                        \
+    "s_mov_b32 m0, %[pos]\n\t"                                                  \
+    "s_add_u32 %[pos], %[pos], 1\n\t"                                           \
+    "v_writelane_b32 %[win], %[last], m0\n\t"                                   \
+    "s_and_b32 %[last], %[q], 0xff\n\t"                                         \
+    "s_branch " BACK "\n" LCARRY ":\n\t"                                       \
+    "s_add_u32 %[last], %[last], 1\n\t"                                         \
+    "s_cmp_eq_u32 %[last], 0xff\n\t"                                            \
+    "s_cbranch_scc0 " L8 "b\n\t"                                                \
+    "s_bitset0_b32 %[c], 27\n" L7 ":\n\t"                                      \
+    "s_lshr_b32 %[q], %[c], 20\n\t"                                             \
+    "s_and_b32 %[c], %[c], 0xfffff\n\t"                                         \
+    "s_mov_b32 %[ct], 7\n\t"                                                    \
+    "s_branch " LCOM "b\n"
 
 // One decision, written as one scalar-unit sequence (mq/mod.rs:213-242 with
 // renormalisation :362-371 and byte-out :373-414): the MPS case that keeps
@@ -712,42 +746,7 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_add_u32 %[c], %[c], %[q]\n\t"
         "s_branch 9f\n"
         "1:\n\t"                      // t >= ct: shift by ct and move a byte out
-        "s_lshl_b32 %[a], %[a], %[ct]\n\t"
-        "s_lshl_b32 %[c], %[c], %[ct]\n\t"
-        "s_sub_u32 %[t], %[t], %[ct]\n\t"
-        "s_cmp_eq_u32 %[last], 0xff\n\t"
-        "s_cbranch_scc1 2f\n\t"
-        "s_bitcmp1_b32 %[c], 27\n\t"
-        "s_cbranch_scc0 3f\n\t"
-        "s_add_u32 %[last], %[last], 1\n\t"
-        "s_cmp_eq_u32 %[last], 0xff\n\t"
-        "s_cbranch_scc0 3f\n\t"
-        "s_bitset0_b32 %[c], 27\n"
-        "2:\n\t"
-        "s_lshr_b32 %[q], %[c], 20\n\t"
-        "s_and_b32 %[c], %[c], 0xfffff\n\t"
-        "s_mov_b32 %[ct], 7\n\t"
-        "s_branch 4f\n"
-        "3:\n\t"
-        "s_lshr_b32 %[q], %[c], 19\n\t"
-        "s_and_b32 %[c], %[c], 0x7ffff\n\t"
-        "s_mov_b32 %[ct], 8\n"
-        // commit the held byte at window index pos - 1 - wbase, hold the new one
-        "4:\n\t"
-        "s_cmp_eq_u32 %[pos], 0\n\t"
-        "s_cbranch_scc1 6f\n\t"
-        "s_sub_u32 %[u], %[pos], %[wb]\n\t"
-        "s_sub_u32 m0, %[u], 1\n\t"
-        "s_cmp_lt_u32 m0, 64\n\t"
-        "s_cbranch_scc0 5f\n\t"
-        "v_writelane_b32 %[w0], %[last], m0\n\t"
-        "s_branch 6f\n"
-        "5:\n\t"
-        "v_writelane_b32 %[w1], %[last], m0\n"
-        "6:\n\t"
-        "s_add_u32 %[pos], %[pos], 1\n\t"
-        "s_and_b32 %[last], %[q], 0xff\n\t"
-        "s_branch 8f\n"
+        MQE_BYTEOUT("2", "3", "5", "6", "8f")
         "7:\n\t"
         // renormalising decision, MPS or LPS, branch-free: conditional
         // exchange to_qe = (a < qe) XOR lps; MPS: a = qe if a < qe else
@@ -778,18 +777,108 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         "s_sub_u32 %[ct], %[ct], %[t]\n"
         "9:"
         : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
-          [cx] "+v"(m.cx), [w0] "+v"(m.w0), [w1] "+v"(m.w1), [m0] "+{m0}"(ctx), [e] "=&s"(e), [q] "=&s"(q),
+          [cx] "+v"(m.cx), [win] "+v"(m.win), [m0] "+{m0}"(ctx), [e] "=&s"(e), [q] "=&s"(q),
           [u] "=&s"(u), [t] "=&s"(t)
-        : [d] "s"(d31), [tab] "v"(m.tab), [wb] "s"(m.wbase), [f16] "s"(0x60010u)
+        : [d] "s"(d31), [tab] "v"(m.tab), [f16] "s"(0x60010u)
         : "scc");
 }
-// Store the window's lower 64 bytes once they are all committed.
+// Coder registers at the start of a block (mq/mod.rs:183-200).  The scalar
+// zeros share one SGPR definition: a constant 0 shared with the ring (a VGPR
+// starting at 0) can be materialised in a VGPR and reach the asm operands
+// through the pass loop's phis, which the backend cannot turn into SGPRs.
+__device__ __forceinline__ void mqe_init(MqE& m, uint32_t mq, uint8_t* out, uint32_t cap) {
+    uint32_t z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    m.a = 0x8000u;
+    m.c = z;
+    m.ct = 12;
+    m.pos = z;
+    m.last = z;
+    m.wbase = z;
+    m.tab = mq;
+    m.win = 0;
+    m.out = out;
+    m.cap = cap;
+    m.overflow = 0;
+}
+// One decision of a 16-decision group (mqe_group16), labels 1<ID><n>.  X is
+// the symbol word, OFF the s_bfe field of its context, DSH the shift putting
+// the decision into bit 31 of DREG.  c += qe is done before the test, so the
+// MPS case that keeps a >= 0x8000 is one taken branch to the next decision;
+// the renormalising path falls through (and takes qe back from c when it
+// exchanges), with the byte-out out of line after the group.
+#define MQE_G_SYM(ID, X, OFF, DSH, DREG)                                           \
+    "s_bfe_u32 m0, %[" X "], " OFF "\n\t" DSH                                     \
+    "v_readlane_b32 %[e], %[cx], m0\n\t"                                         \
+    "s_and_b32 %[q], %[e], 0xffff\n\t"                                           \
+    "s_sub_u32 %[a], %[a], %[q]\n\t"                                             \
+    "s_xor_b32 %[u], %[e], " DREG "\n\t"                                         \
+    "s_lshl_b32 %[t], %[a], 16\n\t"                                              \
+    "s_andn2_b32 %[t], %[t], %[u]\n\t"                                           \
+    "s_add_u32 %[c], %[c], %[q]\n\t"                                             \
+    "s_cmp_lt_i32 %[t], 0\n\t"                                                   \
+    "s_cbranch_scc1 1" ID "1f\n\t"                                               \
+    "s_sub_u32 %[t], %[a], %[q]\n\t"                                             \
+    "s_xor_b32 %[t], %[t], %[u]\n\t"                                             \
+    "s_cmp_lt_i32 %[t], 0\n\t"                                                   \
+    "s_cselect_b32 %[a], %[q], %[a]\n\t"                                         \
+    "s_cselect_b32 %[t], %[q], 0\n\t"                                            \
+    "s_sub_u32 %[c], %[c], %[t]\n\t"                                             \
+    "s_cmp_lt_i32 %[u], 0\n\t"                                                   \
+    "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"                                    \
+    "s_bfe_u32 %[t], %[e], %[t]\n\t"                                             \
+    "v_readlane_b32 %[t], %[tab], %[t]\n\t"                                      \
+    "s_and_b32 %[q], %[e], 0x80000000\n\t"                                       \
+    "s_xor_b32 %[t], %[t], %[q]\n\t"                                             \
+    "v_writelane_b32 %[cx], %[t], m0\n\t"                                        \
+    "s_flbit_i32_b32 %[t], %[a]\n\t"                                             \
+    "s_sub_u32 %[t], %[t], 16\n"                                                  \
+    "1" ID "2:\n\t"                                                              \
+    "s_cmp_lt_u32 %[t], %[ct]\n\t"                                               \
+    "s_cbranch_scc0 1" ID "3f\n\t"                                               \
+    "s_lshl_b32 %[a], %[a], %[t]\n\t"                                            \
+    "s_lshl_b32 %[c], %[c], %[t]\n\t"                                            \
+    "s_sub_u32 %[ct], %[ct], %[t]\n"                                              \
+    "1" ID "1:\n\t"
+#define MQE_G_BO(ID) "1" ID "3:\n\t" MQE_BYTEOUT("1" ID "4", "1" ID "5", "1" ID "6", "1" ID "7", "1" ID "2b")
+#define MQE_G_WORD(I0, I1, I2, I3, X)                                              \
+    MQE_G_SYM(I0, X, "0x50000", "s_lshl_b32 %[d], %[" X "], 24\n\t", "%[d]")     \
+    MQE_G_SYM(I1, X, "0x50008", "s_lshl_b32 %[d], %[" X "], 16\n\t", "%[d]")     \
+    MQE_G_SYM(I2, X, "0x50010", "s_lshl_b32 %[d], %[" X "], 8\n\t", "%[d]")      \
+    MQE_G_SYM(I3, X, "0x50018", "", "%[" X "]")
+
+// 16 decisions, the symbol bytes of words x0..x3 in order (at most 32
+// bytes move out).
+__device__ __forceinline__ void mqe_group16(MqE& m, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+    uint32_t e, q, u, t, d;
+    asm volatile(
+        MQE_G_WORD("00", "01", "02", "03", "x0")
+        MQE_G_WORD("04", "05", "06", "07", "x1")
+        MQE_G_WORD("08", "09", "10", "11", "x2")
+        MQE_G_WORD("12", "13", "14", "15", "x3")
+        "s_branch 9f\n"
+        MQE_G_BO("00") MQE_G_BO("01") MQE_G_BO("02") MQE_G_BO("03")
+        MQE_G_BO("04") MQE_G_BO("05") MQE_G_BO("06") MQE_G_BO("07")
+        MQE_G_BO("08") MQE_G_BO("09") MQE_G_BO("10") MQE_G_BO("11")
+        MQE_G_BO("12") MQE_G_BO("13") MQE_G_BO("14") MQE_G_BO("15")
+        "9:"
+        : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
+          [cx] "+v"(m.cx), [win] "+v"(m.win), [e] "=&s"(e), [q] "=&s"(q), [u] "=&s"(u),
+          [t] "=&s"(t), [d] "=&s"(d)
+        : [x0] "s"(x0), [x1] "s"(x1), [x2] "s"(x2), [x3] "s"(x3), [tab] "v"(m.tab), [f16] "s"(0x60010u)
+        : "scc", "m0");
+}
+// Store ring bytes [wbase, wbase + 32) once they are all committed (byte
+// pos - 1 is still held).
 __device__ __forceinline__ void mqe_spill(MqE& m) {
-    if (m.pos > m.wbase + 64u) {
-        if (m.wbase + 64u <= m.cap) m.out[m.wbase + lid()] = (uint8_t)m.w0;
-        else m.overflow = 1u;
-        m.w0 = m.w1;
-        m.wbase += 64u;
+    if (m.pos > m.wbase + 32u) {
+        const uint32_t k = (lid() - 1u - m.wbase) & 63u;
+        if (m.wbase + 32u <= m.cap) {
+            if (k < 32u) m.out[m.wbase + k] = (uint8_t)m.win;
+        } else {
+            m.overflow = 1u;
+        }
+        m.wbase += 32u;
     }
 }
 // Code the n symbol bytes of one pass from LDS.
@@ -799,16 +888,9 @@ __device__ __forceinline__ void mqe_pass(MqE& m, const uint32_t* symw, uint32_t 
         const uint32_t cnt = min(256u, n - base);
         uint32_t q = 0;
         // 16 symbols move at most 32 bytes out, so one spill check per 16
-        // keeps the 128-byte window from filling (at most 96 bytes pending)
+        // keeps the ring from reusing an unstored byte (at most 64 pending)
         for (; q + 16u <= cnt; q += 16u) {
-#pragma unroll
-            for (uint32_t g = 0; g < 4u; g++) {
-                const uint32_t w = rl(sv, (q >> 2) + g);
-                mqe_symbol(m, w & 31u, w << 24);
-                mqe_symbol(m, (w >> 8) & 31u, w << 16);
-                mqe_symbol(m, (w >> 16) & 31u, w << 8);
-                mqe_symbol(m, (w >> 24) & 31u, w);
-            }
+            mqe_group16(m, rl(sv, q >> 2), rl(sv, (q >> 2) + 1u), rl(sv, (q >> 2) + 2u), rl(sv, (q >> 2) + 3u));
             mqe_spill(m);
         }
         for (; q + 4u <= cnt; q += 4u) {
@@ -850,11 +932,7 @@ __device__ __forceinline__ void mqe_byteout(MqE& m) {
             m.ct = 8;
         }
     }
-    if (m.pos > 0) {
-        const uint32_t idx = m.pos - 1u - m.wbase;
-        if (idx < 64u) m.w0 = wl(m.w0, m.last, idx);
-        else m.w1 = wl(m.w1, m.last, idx - 64u);
-    }
+    m.win = wlane(m.win, m.last, m.pos & 63u);
     m.pos++;
     m.last = b & 0xffu;
 }
@@ -870,22 +948,12 @@ __device__ __forceinline__ uint32_t mqe_flush(MqE& m) {
     m.c <<= m.ct;
     mqe_byteout(m);
     const uint32_t total = m.last == 0xffu ? m.pos - 1u : m.pos;
-    {
-        const uint32_t idx = m.pos - 1u - m.wbase;  // commit the held byte
-        if (idx < 64u) m.w0 = wl(m.w0, m.last, idx);
-        else m.w1 = wl(m.w1, m.last, idx - 64u);
-    }
-    const uint32_t nwin = m.pos - m.wbase;  // <= 128 bytes pending
-    const uint32_t lane = lid();
-    if (lane < nwin) {
-        const uint32_t o = m.wbase + lane;
-        if (o < m.cap) m.out[o] = (uint8_t)m.w0;
-        else m.overflow = 1u;
-    }
-    if (lane + 64u < nwin) {
-        const uint32_t o = m.wbase + 64u + lane;
-        if (o < m.cap) m.out[o] = (uint8_t)m.w1;
-        else m.overflow = 1u;
+    m.win = wlane(m.win, m.last, m.pos & 63u);  // commit the held byte
+    const uint32_t nwin = m.pos - m.wbase;    // <= 34 bytes pending
+    const uint32_t k = (lid() - 1u - m.wbase) & 63u;
+    if (k < nwin) {
+        const uint32_t o = m.wbase + k;
+        if (o < m.cap) m.out[o] = (uint8_t)m.win;
     }
     if (m.wbase + nwin > m.cap) m.overflow = 1u;
     return total;
@@ -933,23 +1001,97 @@ __device__ __forceinline__ int load_block(const BlockJob& J, uint32_t (&mag)[64]
     return mbp;
 }
 
-// Tier-1 encoder: one wavefront per code-block (blockIdx.x = job index).
-// Outputs the terminated segment into out + job.out_off, the per-pass byte
-// snapshots and newly-significant counts, and the block statistics.
+// ---- wave-coder schedule ----------------------------------------------------
+// A SIMD runs about two coder waves at once, and the kernel lasts as long as
+// its most loaded SIMD (a block with many bit-planes runs at raised issue
+// priority and starves the wave beside it).  So blocks are handed out in
+// descending estimated cost: the first wave to start on a SIMD takes the next
+// heavy block, any later wave there the next light one.  sched holds
+// cost[n] | order[n] | heavy count, light count | arrivals per SIMD.
+constexpr uint32_t kSimdSlots = 1u << 14;  // xcc(4) se(3) sh(1) cu(4) simd(2)
+
+// Estimated cost of a block in MQ decisions: fitted on document and photo
+// pages (correlation 0.98 with the decisions coded), plus about 470
+// decisions' worth of context formation per bit-plane.
+__global__ __launch_bounds__(64) void t1_cost_kernel(const BlockJob* __restrict__ jobs, uint32_t* __restrict__ sched) {
+    const BlockJob J = jobs[blockIdx.x];
+    uint32_t mag[64];
+    uint64_t neg;
+    BlockResult R;
+    const int mbp = load_block(J, mag, neg, R);
+    uint32_t bits = 0, z = 0;
+#pragma unroll
+    for (int y = 0; y < 64; y++) {
+        const uint32_t bl = mag[y] ? 32u - (uint32_t)__builtin_clz(mag[y]) : 0u;
+        if (y < J.h && (int)lid() < J.w) {
+            bits += bl;
+            z += (uint32_t)mbp - bl;
+        }
+    }
+    bits = wave_sum(bits);
+    z = wave_sum(z);
+    if (lid() == 0) sched[blockIdx.x] = mbp ? (9u * bits + 4u * z + 12u * R.nnz) / 8u + 470u * (uint32_t)mbp : 0u;
+}
+// Counting sort of the blocks by descending cost (512-decision buckets) and
+// reset of the hand-out counters.  One workgroup.
+__global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t* __restrict__ sched) {
+    __shared__ uint32_t hist[256];
+    const int tid = threadIdx.x;
+    if (tid < 256) hist[tid] = 0;
+    uint32_t* ctrl = sched + 2 * (size_t)n;
+    for (uint32_t i = tid; i < 2u + kSimdSlots; i += 1024u) ctrl[i] = 0;
+    __syncthreads();
+    for (int b = tid; b < n; b += 1024) atomicAdd(&hist[255u - min(255u, sched[b] >> 9)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int k = 0; k < 256; k++) {
+            const uint32_t c = hist[k];
+            hist[k] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    for (int b = tid; b < n; b += 1024) sched[n + atomicAdd(&hist[255u - min(255u, sched[b] >> 9)], 1u)] = (uint32_t)b;
+}
+// The block this wave codes.  The n waves of the launch take n distinct
+// positions of the order (heavy ones from the front, light ones from the
+// back), wherever they are placed, so every block is coded exactly once.
+__device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched) {
+    uint32_t b = 0;
+    if (lid() == 0) {
+        uint32_t hw, xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)\n\ts_getreg_b32 %1, hwreg(HW_REG_XCC_ID)" : "=s"(hw), "=s"(xcc));
+        const uint32_t simd = ((xcc & 15u) << 10) | (((hw >> 13) & 7u) << 7) | (((hw >> 12) & 1u) << 6) |
+                              (((hw >> 8) & 15u) << 2) | ((hw >> 4) & 3u);
+        uint32_t* ctrl = sched + 2 * (size_t)n;
+        const bool first = atomicAdd(&ctrl[2 + simd], 1u) == 0u;
+        const uint32_t k = atomicAdd(&ctrl[first ? 0 : 1], 1u);
+        b = sched[n + (first ? k : n - 1u - k)];
+    }
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+}
+
+// Tier-1 encoder: one wavefront per code-block (the block from
+// t1_pick_block).  Outputs the terminated segment into out + job.out_off,
+// the per-pass byte snapshots and newly-significant counts, and the block
+// statistics.
 __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                        uint8_t* __restrict__ out, BlockResult* __restrict__ results,
-                                                       uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs) {
+                                                       uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs,
+                                                       uint32_t* __restrict__ sched) {
     __shared__ uint32_t symw[kSymCap / 4];
     uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
     const int lane = lid();
-    const BlockJob J = jobs[blockIdx.x];
+    const uint32_t blk = t1_pick_block(gridDim.x, sched);
+    const BlockJob J = jobs[blk];
     const int w = J.w, h = J.h;
     uint32_t mag[64];
     uint64_t neg;
     BlockResult R;
     const int mbp = load_block(J, mag, neg, R);
     if (mbp == 0) {
-        if (lane == 0) results[blockIdx.x] = R;
+        if (lane == 0) results[blk] = R;
         return;
     }
     // The kernel ends with its heaviest block; blocks with many bit-planes
@@ -958,20 +1100,9 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     Tables t;
     load_tables(t, tabs, J.band);
     MqE m;
-    m.a = 0x8000u;
-    m.c = 0;
-    m.ct = 12;
-    m.pos = 0;
-    m.last = 0;
-    m.tab = t.mq;
+    mqe_init(m, t.mq, out + J.out_off, J.out_cap);
     // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
     m.cx = (uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64);
-    m.w0 = 0;
-    m.w1 = 0;
-    m.wbase = 0;
-    m.out = out + J.out_off;
-    m.cap = J.out_cap;
-    m.overflow = 0;
     Col st = {0, 0, 0, 0};
     uint32_t snap0 = 0, snap1 = 0, ns0 = 0, ns1 = 0;
     uint32_t np = 0;
@@ -1014,8 +1145,8 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     R.nbytes = total;
     R.npasses = (uint8_t)np;
     R.overflow = (uint8_t)(m.overflow ? 1 : 0);
-    uint32_t* sp = snaps + (size_t)blockIdx.x * kMaxPasses;
-    uint16_t* nsp = newsigs + (size_t)blockIdx.x * kMaxPasses;
+    uint32_t* sp = snaps + (size_t)blk * kMaxPasses;
+    uint16_t* nsp = newsigs + (size_t)blk * kMaxPasses;
     if ((uint32_t)lane < np) {
         sp[lane] = snap0;
         nsp[lane] = (uint16_t)ns0;
@@ -1024,7 +1155,7 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
         sp[lane + 64] = snap1;
         nsp[lane + 64] = (uint16_t)ns1;
     }
-    if (lane == 0) results[blockIdx.x] = R;
+    if (lane == 0) results[blk] = R;
 }
 
 // ---- batched Tier-1 encoder: context formation, then one lane per block ------
@@ -1321,21 +1452,10 @@ __device__ __forceinline__ void stream_mq(int b, const BlockJob* __restrict__ jo
     Tables t;
     load_tables(t, tabs, J.band);
     MqE m;
-    m.a = 0x8000u;
-    m.c = 0;
-    m.ct = 12;
-    m.pos = 0;
-    m.last = 0;
-    m.tab = t.mq;
+    mqe_init(m, t.mq, out + J.out_off, J.out_cap);
     // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
     m.cx = (uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64);
     if (lane == kPad || lane == kMarker) m.cx = 0;
-    m.w0 = 0;
-    m.w1 = 0;
-    m.wbase = 0;
-    m.out = out + J.out_off;
-    m.cap = J.out_cap;
-    m.overflow = 0;
     const uint32_t* words = reinterpret_cast<const uint32_t*>(pool + ((uint64_t)soff16[b] << 4));
     const uint32_t* pe = pend + (size_t)b * kMaxPasses;
     const uint32_t pe0 = lane < np ? pe[lane] >> 2 : 0u;  // pass ends in words
@@ -1598,9 +1718,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
     }
 }
 
+size_t t1_sched_words(int n) { return 2 * (size_t)n + 2 + kSimdSlots; }
 void launch_t1_encode(const BlockJob* jobs, int n, const uint32_t* tabs, uint8_t* out, BlockResult* res, uint32_t* snaps,
-                      uint16_t* newsigs, hipStream_t st) {
-    if (n > 0) t1_encode_kernel<<<n, 64, 0, st>>>(jobs, tabs, out, res, snaps, newsigs);
+                      uint16_t* newsigs, uint32_t* sched, hipStream_t st) {
+    if (n <= 0) return;
+    t1_cost_kernel<<<n, 64, 0, st>>>(jobs, sched);
+    t1_order_kernel<<<1, 1024, 0, st>>>(n, sched);
+    t1_encode_kernel<<<n, 64, 0, st>>>(jobs, tabs, out, res, snaps, newsigs, sched);
 }
 void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResult* res, uint32_t* pend, uint16_t* newsigs,
                     uint32_t* soff16, uint8_t* pool, unsigned long long* pool_used, unsigned long long pool_cap,
