@@ -231,8 +231,8 @@ int zc_index_context(int orient, int idx) {
 // states (0 -> 1, 6 -> 6, 14 -> 14) get copies at words 47..49 with bit 31
 // set, and nlps' points there, so the next state of any decision is
 // word[field] XOR (MPS in bit 31): the copy's bit 31 flips the MPS.
-std::array<uint32_t, 384> make_t1_tables() {
-    std::array<uint32_t, 384> t{};
+std::array<uint32_t, 448> make_t1_tables() {
+    std::array<uint32_t, 448> t{};
     auto word = [](int i) {
         const int l = kNlps[i];
         const uint32_t nl = kSwitch(i) ? (l == 1 ? 47u : l == 6 ? 48u : 49u) : (uint32_t)l;
@@ -248,6 +248,16 @@ std::array<uint32_t, 384> make_t1_tables() {
         int ctx, xr;
         sign_context(i, ctx, xr);
         t[320 + i / 4] |= (uint32_t)(ctx | (xr << 4)) << ((i % 4) * 8);
+    }
+    // Words 384..447: the same sign entries indexed by the decoder's layout,
+    // significance of N, W, E, S in bits 1, 3, 5, 7 (as above) and their
+    // signs in bits 0, 2, 4, 6.
+    for (int i = 0; i < 256; i++) {
+        const int old = (i & 0xAA) | (((i >> 0) & 1) << 4) | (((i >> 2) & 1) << 0) | (((i >> 4) & 1) << 2) |
+                        (((i >> 6) & 1) << 6);
+        int ctx, xr;
+        sign_context(old, ctx, xr);
+        t[384 + i / 4] |= (uint32_t)(ctx | (xr << 4)) << ((i % 4) * 8);
     }
     return t;
 }

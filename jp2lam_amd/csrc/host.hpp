@@ -52,7 +52,7 @@ Rect band_rect(uint32_t w, uint32_t h, int levels, int res, int band);
 
 // Kernel lookup tables (MQ states, ZC/SC contexts) in the per-lane layout
 // the Tier-1 kernels read: 64 MQ words, 4 x 64 ZC words, 64 SC words.
-std::array<uint32_t, 384> make_t1_tables();
+std::array<uint32_t, 448> make_t1_tables();
 // Plain tables for tests: MQ expanded to 94 states, ZC [4][512], SC/SPB [256].
 void expanded_tables(uint16_t* qe, uint8_t* mps, uint8_t* nmps, uint8_t* nlps, uint8_t* zc, uint8_t* sc,
                      uint8_t* spb);

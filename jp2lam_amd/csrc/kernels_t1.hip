@@ -266,177 +266,6 @@ struct Coder<true> {
     __device__ __forceinline__ void column() { dec_slide(m); }
 };
 
-// Significance propagation pass over the block (t1.rs:805-905,
-// decode/t1.rs:211-266).  `bits` = magnitude bit-plane masks (encoder) and
-// `negs` = sign masks (encoder); the decoder accumulates decoded plane bits
-// into `plane`.  Returns the number of newly significant samples.
-template <bool DEC>
-__device__ uint32_t sp_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t negs, uint64_t& plane,
-                            int w, int h) {
-    const int lane = lid();
-    uint32_t newsig = 0;
-    const int nstripes = (h + 3) >> 2;
-    for (int s = 0; s < nstripes; s++) {
-        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
-        const uint32_t rmask = (1u << nrows) - 1u;
-        uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
-        uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
-        uint32_t P = sig6 | (sgn6 << 6) | (b4 << 12) | (n4 << 16);
-        // static candidates: not significant with a significant 8-neighbour
-        uint32_t w6 = __shfl_up(sig6, 1, 64), e6 = __shfl_down(sig6, 1, 64);
-        if (lane == 0) w6 = 0;
-        if (lane == 63) e6 = 0;
-        uint32_t o6 = w6 | sig6 | e6;
-        uint32_t nb4 = (o6 | (o6 >> 1) | (o6 >> 2)) & 15u;
-        uint32_t cand = (~(sig6 >> 1)) & nb4 & rmask;
-        uint64_t M = ballot(lane < w && cand != 0);
-        uint32_t upd = 0;
-        uint32_t newW = 0, newWs = 0;
-        int x = M ? __builtin_ctzll(M) : 64;
-        while (x < w) {
-            cd.column();
-            uint32_t Pc = rl(P, x), Pw = x > 0 ? rl(P, x - 1) : 0u, Pe = x < 63 ? rl(P, x + 1) : 0u;
-            uint32_t sigC = Pc & 63u, sgnC = (Pc >> 6) & 63u, bC = (Pc >> 12) & 15u, nC = (Pc >> 16) & 15u;
-            uint32_t sigW = (Pw & 63u) | (newW << 1), sgnW = ((Pw >> 6) & 63u) | (newWs << 1);
-            uint32_t sigE = Pe & 63u, sgnE = (Pe >> 6) & 63u;
-            uint32_t newC = 0, newCs = 0, visC = 0;
-#pragma unroll
-            for (int ci = 0; ci < 4; ci++) {
-                if (ci >= nrows) break;
-                const uint32_t r = (uint32_t)ci + 1;
-                if ((sigC >> r) & 1u) continue;
-                uint32_t effC = sigC | (newC << 1);
-                uint32_t nb = nbmask(sigW, effC, sigE, r);
-                if (!nb) continue;
-                uint32_t bit = cd.code(t, zc_ctx(t, nb), (bC >> ci) & 1u);
-                visC |= 1u << ci;
-                if (bit) {
-                    uint32_t effS = sgnC | (newCs << 1);
-                    uint32_t e = sc_entry(t, signidx(sigW, sgnW, effC, effS, sigE, sgnE, r));
-                    uint32_t sg = cd.code(t, e & 15u, ((nC >> ci) & 1u) ^ (e >> 4)) ^ (e >> 4);
-                    newC |= 1u << ci;
-                    newCs |= sg << ci;
-                    newsig++;
-                }
-            }
-            upd = wl(upd, newC | (newCs << 4) | (visC << 8), x);
-            uint64_t rest = x < 63 ? (M >> (x + 1)) << (x + 1) : 0ull;
-            int nx = rest ? __builtin_ctzll(rest) : 64;
-            if (newC && x + 1 < w) nx = x + 1;
-            newW = nx == x + 1 ? newC : 0u;
-            newWs = nx == x + 1 ? newCs : 0u;
-            x = nx;
-        }
-        st.sig |= (uint64_t)(upd & 15u) << (4 * s);
-        st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
-        st.vis |= (uint64_t)((upd >> 8) & 15u) << (4 * s);
-        if (DEC) plane |= (uint64_t)(upd & 15u) << (4 * s);
-    }
-    return newsig;
-}
-
-// Cleanup pass with run-length mode on full clean stripes (t1.rs:977-1197,
-// decode/t1.rs:317-446).
-template <bool DEC>
-__device__ uint32_t cl_pass(Coder<DEC>& cd, const Tables& t, Col& st, uint64_t bits, uint64_t negs, uint64_t& plane,
-                            int w, int h) {
-    const int lane = lid();
-    uint32_t newsig = 0;
-    const int nstripes = (h + 3) >> 2;
-    for (int s = 0; s < nstripes; s++) {
-        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
-        const uint32_t rmask = (1u << nrows) - 1u;
-        uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
-        uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
-        uint32_t b4 = (uint32_t)(bits >> (4 * s)) & 15u, n4 = (uint32_t)(negs >> (4 * s)) & 15u;
-        uint32_t w6 = __shfl_up(sig6, 1, 64), e6 = __shfl_down(sig6, 1, 64);
-        if (lane == 0) w6 = 0;
-        if (lane == 63) e6 = 0;
-        // static part of stripe_is_clean (flags.rs:202-215): own rows neither
-        // significant nor visited, no significant neighbour in W/own/E windows
-        uint32_t sclean = (nrows == 4 && sig6 == 0 && vis4 == 0 && w6 == 0 && e6 == 0) ? 1u : 0u;
-        uint32_t P = sig6 | (sgn6 << 6) | (b4 << 12) | (n4 << 16) | (vis4 << 20) | (sclean << 24);
-        uint32_t todo = ~((sig6 >> 1) | vis4) & rmask;
-        uint64_t M = ballot(lane < w && todo != 0);
-        uint32_t upd = 0;
-        uint32_t newW = 0, newWs = 0;
-        int prevx = -2;
-        while (M) {
-            int x = __builtin_ctzll(M);
-            M &= M - 1;
-            if (x != prevx + 1) {
-                newW = 0;
-                newWs = 0;
-            }
-            cd.column();
-            uint32_t Pc = rl(P, x), Pw = x > 0 ? rl(P, x - 1) : 0u, Pe = x < 63 ? rl(P, x + 1) : 0u;
-            uint32_t sigC = Pc & 63u, sgnC = (Pc >> 6) & 63u, bC = (Pc >> 12) & 15u, nC = (Pc >> 16) & 15u;
-            uint32_t visC = (Pc >> 20) & 15u;
-            uint32_t sigW = (Pw & 63u) | (newW << 1), sgnW = ((Pw >> 6) & 63u) | (newWs << 1);
-            uint32_t sigE = Pe & 63u, sgnE = (Pe >> 6) & 63u;
-            uint32_t newC = 0, newCs = 0;
-            int ci0 = 0;
-            if (((Pc >> 24) & 1u) && newW == 0) {
-                // run mode
-                uint32_t run;
-                if (DEC) {
-                    run = cd.code(t, CTX_AGG, 0);
-                } else {
-                    run = bC ? 1u : 0u;
-                    cd.code(t, CTX_AGG, run);
-                }
-                if (!run) {
-                    ci0 = 4;
-                } else {
-                    uint32_t rlen;
-                    if (DEC) {
-                        uint32_t hi = cd.code(t, CTX_UNI, 0);
-                        uint32_t lo = cd.code(t, CTX_UNI, 0);
-                        rlen = (hi << 1) | lo;
-                    } else {
-                        rlen = (uint32_t)__builtin_ctz(bC);
-                        cd.code(t, CTX_UNI, (rlen >> 1) & 1u);
-                        cd.code(t, CTX_UNI, rlen & 1u);
-                    }
-                    const uint32_t r = rlen + 1;
-                    uint32_t e = sc_entry(t, signidx(sigW, sgnW, sigC, sgnC, sigE, sgnE, r));
-                    uint32_t sg = cd.code(t, e & 15u, ((nC >> rlen) & 1u) ^ (e >> 4)) ^ (e >> 4);
-                    newC = 1u << rlen;
-                    newCs = sg << rlen;
-                    newsig++;
-                    ci0 = (int)rlen + 1;
-                }
-            }
-#pragma unroll
-            for (int ci = 0; ci < 4; ci++) {
-                if (ci >= nrows) break;
-                if (ci < ci0) continue;
-                const uint32_t r = (uint32_t)ci + 1;
-                if (((sigC >> r) & 1u) || ((visC >> ci) & 1u)) continue;
-                uint32_t effC = sigC | (newC << 1);
-                uint32_t nb = nbmask(sigW, effC, sigE, r);
-                uint32_t bit = cd.code(t, nb ? zc_ctx(t, nb) : (uint32_t)CTX_ZC, (bC >> ci) & 1u);
-                if (bit) {
-                    uint32_t effS = sgnC | (newCs << 1);
-                    uint32_t e = sc_entry(t, signidx(sigW, sgnW, effC, effS, sigE, sgnE, r));
-                    uint32_t sg = cd.code(t, e & 15u, ((nC >> ci) & 1u) ^ (e >> 4)) ^ (e >> 4);
-                    newC |= 1u << ci;
-                    newCs |= sg << ci;
-                    newsig++;
-                }
-            }
-            if (newC) upd = wl(upd, newC | (newCs << 4), x);
-            newW = newC;
-            newWs = newCs;
-            prevx = x;
-        }
-        st.sig |= (uint64_t)(upd & 15u) << (4 * s);
-        st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
-        if (DEC) plane |= (uint64_t)(upd & 15u) << (4 * s);
-    }
-    return newsig;
-}
-
 __device__ __forceinline__ void load_tables(Tables& t, const uint32_t* __restrict__ tabs, int band) {
     const int lane = lid();
     t.mq = tabs[lane];
@@ -1550,6 +1379,174 @@ __global__ __launch_bounds__(64) void t1_hybrid_kernel(int nb, const BlockJob* _
     }
 }
 
+// ---- decoder significance propagation and cleanup passes ----------------------
+// The scan keeps, per column of the current stripe, X: the significance of
+// the west, own and east columns interleaved, bits 3k, 3k + 1, 3k + 2 for
+// window row k (row 4s - 1 + k), and Y: their signs in the same places.
+// Row ci's neighbourhood is then (X >> 3ci) & 0x1ff, bits NW N NE W C E SW S
+// SE: the ZC table index of tier1/consts.rs:1-27 with the own bit (C) in the
+// middle; its N W E S significance (bits 1 3 5 7) and the matching sign bits
+// of Y shifted down by one form the index of the second sign table.  Both
+// words come lane-parallel from the state at the start of the stripe; the
+// scan adds what it decodes (own rows below, the next column's west bits).
+
+// spr lane v = v (six bits) with bit k moved to bit 3k.
+__device__ __forceinline__ uint32_t spread_lut() {
+    const uint32_t v = lid();
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6) | ((v & 16u) << 8) | ((v & 32u) << 10);
+}
+// Interleaved windows of one stripe: lane x gets column x's word.
+__device__ __forceinline__ uint32_t interleave3(uint32_t spr, uint32_t own6) {
+    const uint32_t sp = (uint32_t)__shfl((int)spr, (int)own6, 64);
+    return from_west(sp) | (sp << 1) | (from_east(sp) << 2);
+}
+__device__ __forceinline__ uint32_t sc2_entry(uint32_t sc2, uint32_t si) { return (rl(sc2, si >> 2) >> ((si & 3u) * 8u)) & 255u; }
+// The sign decision of a row that became significant (decode/t1.rs sign
+// coding with the context of tier1/helpers.rs:53-89); returns the sign.
+__device__ __forceinline__ uint32_t dec_sign(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t nb, uint32_t Yc,
+                                            uint32_t ci) {
+    const uint32_t si = (nb & 0xAAu) | (((Yc >> (3u * ci)) & 0xAAu) >> 1);
+    const uint32_t e = sc2_entry(sc2, si);
+    return cd.code(t, e & 15u, 0) ^ (e >> 4);
+}
+
+// Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).
+__device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
+                            int w, int h) {
+    const int lane = lid();
+    const int nstripes = (h + 3) >> 2;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
+        const uint32_t rmask = (1u << nrows) - 1u;
+        const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
+        const uint32_t X = interleave3(spr, sig6), Y = interleave3(spr, sgn6);
+        // static candidates: not significant with a significant 8-neighbour
+        const uint32_t o6 = from_west(sig6) | sig6 | from_east(sig6);
+        const uint32_t nb4 = (o6 | (o6 >> 1) | (o6 >> 2)) & 15u;
+        const uint32_t cand = (~(sig6 >> 1)) & nb4 & rmask;
+        const uint64_t M = ballot(lane < w && cand != 0);
+        uint32_t upd = 0, nWX = 0, nWY = 0;
+        int x = M ? __builtin_ctzll(M) : 64;
+        while (x < w) {
+            cd.column();
+            uint32_t Xc = rl(X, x) | nWX, Yc = rl(Y, x) | nWY;
+            uint32_t newC = 0, newCs = 0, visC = 0;
+            nWX = 0;
+            nWY = 0;
+#pragma unroll
+            for (int ci = 0; ci < 4; ci++) {
+                if (ci >= nrows) break;
+                const uint32_t nb = (Xc >> (3 * ci)) & 0x1ffu;
+                if ((nb & 0x10u) || nb == 0) continue;  // significant, or no significant neighbour
+                const uint32_t bit = cd.code(t, zc_ctx(t, nb), 0);
+                visC |= 1u << ci;
+                if (bit) {
+                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, (uint32_t)ci);
+                    Xc |= 1u << (3 * ci + 4);
+                    Yc |= sg << (3 * ci + 4);
+                    nWX |= 1u << (3 * ci + 3);
+                    nWY |= sg << (3 * ci + 3);
+                    newC |= 1u << ci;
+                    newCs |= sg << ci;
+                }
+            }
+            upd = wl(upd, newC | (newCs << 4) | (visC << 8), x);
+            const uint64_t rest = x < 63 ? (M >> (x + 1)) << (x + 1) : 0ull;
+            int nx = rest ? __builtin_ctzll(rest) : 64;
+            if (newC && x + 1 < w) nx = x + 1;
+            if (nx != x + 1) {
+                nWX = 0;
+                nWY = 0;
+            }
+            x = nx;
+        }
+        st.sig |= (uint64_t)(upd & 15u) << (4 * s);
+        st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
+        st.vis |= (uint64_t)((upd >> 8) & 15u) << (4 * s);
+        plane |= (uint64_t)(upd & 15u) << (4 * s);
+    }
+}
+
+// Cleanup pass with run-length mode on full clean stripes (t1.rs:977-1197,
+// decode/t1.rs:317-446).
+__device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
+                            int w, int h) {
+    const int lane = lid();
+    const int nstripes = (h + 3) >> 2;
+    for (int s = 0; s < nstripes; s++) {
+        const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
+        const uint32_t rmask = (1u << nrows) - 1u;
+        const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
+        const uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
+        const uint32_t X = interleave3(spr, sig6), Y = interleave3(spr, sgn6);
+        // static part of stripe_is_clean (flags.rs:202-215): own rows neither
+        // significant nor visited, no significant neighbour in W/own/E windows
+        const uint32_t sclean = (nrows == 4 && X == 0 && vis4 == 0) ? 1u : 0u;
+        const uint32_t PX = X | (vis4 << 18) | (sclean << 22);
+        const uint32_t todo = ~((sig6 >> 1) | vis4) & rmask;
+        uint64_t M = ballot(lane < w && todo != 0);
+        uint32_t upd = 0, nWX = 0, nWY = 0;
+        int prevx = -2;
+        while (M) {
+            const int x = __builtin_ctzll(M);
+            M &= M - 1;
+            if (x != prevx + 1) {
+                nWX = 0;
+                nWY = 0;
+            }
+            cd.column();
+            const uint32_t Pc = rl(PX, x);
+            const uint32_t visC = (Pc >> 18) & 15u;
+            uint32_t Xc = (Pc & 0x3ffffu) | nWX, Yc = rl(Y, x) | nWY;
+            const bool run_mode = ((Pc >> 22) & 1u) && nWX == 0;
+            uint32_t newC = 0, newCs = 0;
+            nWX = 0;
+            nWY = 0;
+            int ci0 = 0;
+            if (run_mode) {
+                if (!cd.code(t, CTX_AGG, 0)) {
+                    ci0 = 4;
+                } else {
+                    const uint32_t hi = cd.code(t, CTX_UNI, 0);
+                    const uint32_t lo = cd.code(t, CTX_UNI, 0);
+                    const uint32_t rlen = (hi << 1) | lo;
+                    // a clean stripe has no significant neighbour: sign index 0
+                    const uint32_t sg = dec_sign(cd, t, sc2, 0u, 0u, 0u);
+                    Xc |= 1u << (3 * rlen + 4);
+                    Yc |= sg << (3 * rlen + 4);
+                    nWX |= 1u << (3 * rlen + 3);
+                    nWY |= sg << (3 * rlen + 3);
+                    newC = 1u << rlen;
+                    newCs = sg << rlen;
+                    ci0 = (int)rlen + 1;
+                }
+            }
+#pragma unroll
+            for (int ci = 0; ci < 4; ci++) {
+                if (ci >= nrows) break;
+                if (ci < ci0) continue;
+                const uint32_t nb = (Xc >> (3 * ci)) & 0x1ffu;
+                if ((nb & 0x10u) || ((visC >> ci) & 1u)) continue;
+                const uint32_t bit = cd.code(t, nb ? zc_ctx(t, nb) : (uint32_t)CTX_ZC, 0);
+                if (bit) {
+                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, (uint32_t)ci);
+                    Xc |= 1u << (3 * ci + 4);
+                    Yc |= sg << (3 * ci + 4);
+                    nWX |= 1u << (3 * ci + 3);
+                    nWY |= sg << (3 * ci + 3);
+                    newC |= 1u << ci;
+                    newCs |= sg << ci;
+                }
+            }
+            if (newC) upd = wl(upd, newC | (newCs << 4), x);
+            prevx = x;
+        }
+        st.sig |= (uint64_t)(upd & 15u) << (4 * s);
+        st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
+        plane |= (uint64_t)(upd & 15u) << (4 * s);
+    }
+}
+
 // Magnitude refinement pass of the decoder (t1.rs:914-965, decode/t1.rs:268-315).
 // Members and contexts do not depend on the pass's own decisions, so the
 // pass is formed lane-parallel into LDS in scan order (as the encoder forms
@@ -1675,6 +1672,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
     if (J.zbp >= 0 && J.passes > 0) {
         Tables t;
         load_tables(t, tabs, J.band);
+        const uint32_t sc2 = tabs[384 + lane];
+        const uint32_t spr = spread_lut();
         Coder<true> cd;
         cd.m.data = payload + J.data_off;
         cd.m.len = J.data_len;
@@ -1688,12 +1687,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
         for (int p = mbp - 1; p >= 0 && remaining > 0; p--) {
             uint64_t plane = 0;
             if (p != mbp - 1) {
-                sp_pass<true>(cd, t, st, 0, 0, plane, w, h);
+                sp_pass_dec(cd, t, sc2, spr, st, plane, w, h);
                 if (--remaining == 0) goto plane_done;
                 mr_pass_dec(cd, t, st, plane, w, h, symw, bitw);
                 if (--remaining == 0) goto plane_done;
             }
-            cl_pass<true>(cd, t, st, 0, 0, plane, w, h);
+            cl_pass_dec(cd, t, sc2, spr, st, plane, w, h);
             remaining--;
         plane_done:
 #pragma unroll
