@@ -1410,7 +1410,11 @@ __device__ __forceinline__ uint32_t dec_sign(Coder<true>& cd, const Tables& t, u
     return cd.code(t, e & 15u, 0) ^ (e >> 4);
 }
 
-// Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).
+// Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).  The
+// column word also carries the rows that can be coded (insignificant with a
+// significant neighbour at the start of the stripe); the scan visits only
+// those, plus the row below one that becomes significant and, in the column
+// after one with new significance, the rows next to it.
 __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
                             int w, int h) {
     const int lane = lid();
@@ -1420,44 +1424,51 @@ __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         const uint32_t rmask = (1u << nrows) - 1u;
         const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
         const uint32_t X = interleave3(spr, sig6), Y = interleave3(spr, sgn6);
-        // static candidates: not significant with a significant 8-neighbour
         const uint32_t o6 = from_west(sig6) | sig6 | from_east(sig6);
-        const uint32_t nb4 = (o6 | (o6 >> 1) | (o6 >> 2)) & 15u;
-        const uint32_t cand = (~(sig6 >> 1)) & nb4 & rmask;
-        const uint64_t M = ballot(lane < w && cand != 0);
-        uint32_t upd = 0, nWX = 0, nWY = 0;
+        const uint32_t insig4 = (~(sig6 >> 1)) & rmask;
+        const uint32_t cand = insig4 & (o6 | (o6 >> 1) | (o6 >> 2));
+        const uint32_t PX = X | (cand << 18) | (insig4 << 22);
+        uint64_t M = ballot(lane < w && cand != 0);
+        uint32_t upd = 0, nWX = 0, nWY = 0, newW = 0;
         int x = M ? __builtin_ctzll(M) : 64;
         while (x < w) {
+            M &= ~(1ull << x);
             cd.column();
-            uint32_t Xc = rl(X, x) | nWX, Yc = rl(Y, x) | nWY;
-            uint32_t newC = 0, newCs = 0, visC = 0;
-            nWX = 0;
-            nWY = 0;
-#pragma unroll
-            for (int ci = 0; ci < 4; ci++) {
-                if (ci >= nrows) break;
-                const uint32_t nb = (Xc >> (3 * ci)) & 0x1ffu;
-                if ((nb & 0x10u) || nb == 0) continue;  // significant, or no significant neighbour
+            const uint32_t Pc = rl(PX, x);
+            const uint32_t insig = Pc >> 22;
+            uint32_t Xc = (Pc & 0x3ffffu) | nWX;
+            uint32_t visit = ((Pc >> 18) & 15u) | (insig & (newW | (newW << 1) | (newW >> 1)));
+            uint32_t Yc = 0;
+            bool have_y = false;
+            uint32_t newC = 0, newCs = 0, visC = 0, nX = 0, nY = 0;
+            while (visit) {
+                const uint32_t ci = (uint32_t)__builtin_ctz(visit);
+                visit &= visit - 1u;
+                const uint32_t nb = (Xc >> (3u * ci)) & 0x1ffu;
                 const uint32_t bit = cd.code(t, zc_ctx(t, nb), 0);
                 visC |= 1u << ci;
                 if (bit) {
-                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, (uint32_t)ci);
-                    Xc |= 1u << (3 * ci + 4);
-                    Yc |= sg << (3 * ci + 4);
-                    nWX |= 1u << (3 * ci + 3);
-                    nWY |= sg << (3 * ci + 3);
+                    if (!have_y) {
+                        Yc = rl(Y, x) | nWY;
+                        have_y = true;
+                    }
+                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, ci);
+                    Xc |= 1u << (3u * ci + 4u);
+                    Yc |= sg << (3u * ci + 4u);
+                    nX |= 1u << (3u * ci + 3u);
+                    nY |= sg << (3u * ci + 3u);
                     newC |= 1u << ci;
                     newCs |= sg << ci;
+                    visit |= insig & (2u << ci);  // the row below gains a significant neighbour
                 }
             }
             upd = wl(upd, newC | (newCs << 4) | (visC << 8), x);
-            const uint64_t rest = x < 63 ? (M >> (x + 1)) << (x + 1) : 0ull;
-            int nx = rest ? __builtin_ctzll(rest) : 64;
+            int nx = M ? __builtin_ctzll(M) : 64;
             if (newC && x + 1 < w) nx = x + 1;
-            if (nx != x + 1) {
-                nWX = 0;
-                nWY = 0;
-            }
+            const bool next = nx == x + 1;
+            nWX = next ? nX : 0u;
+            nWY = next ? nY : 0u;
+            newW = next ? newC : 0u;
             x = nx;
         }
         st.sig |= (uint64_t)(upd & 15u) << (4 * s);
@@ -1468,7 +1479,8 @@ __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
 }
 
 // Cleanup pass with run-length mode on full clean stripes (t1.rs:977-1197,
-// decode/t1.rs:317-446).
+// decode/t1.rs:317-446).  Every row neither significant nor visited is
+// coded; the column word carries those rows.
 __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
                             int w, int h) {
     const int lane = lid();
@@ -1482,8 +1494,8 @@ __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         // static part of stripe_is_clean (flags.rs:202-215): own rows neither
         // significant nor visited, no significant neighbour in W/own/E windows
         const uint32_t sclean = (nrows == 4 && X == 0 && vis4 == 0) ? 1u : 0u;
-        const uint32_t PX = X | (vis4 << 18) | (sclean << 22);
         const uint32_t todo = ~((sig6 >> 1) | vis4) & rmask;
+        const uint32_t PX = X | (todo << 18) | (sclean << 22);
         uint64_t M = ballot(lane < w && todo != 0);
         uint32_t upd = 0, nWX = 0, nWY = 0;
         int prevx = -2;
@@ -1496,49 +1508,52 @@ __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
             }
             cd.column();
             const uint32_t Pc = rl(PX, x);
-            const uint32_t visC = (Pc >> 18) & 15u;
-            uint32_t Xc = (Pc & 0x3ffffu) | nWX, Yc = rl(Y, x) | nWY;
-            const bool run_mode = ((Pc >> 22) & 1u) && nWX == 0;
-            uint32_t newC = 0, newCs = 0;
-            nWX = 0;
-            nWY = 0;
-            int ci0 = 0;
-            if (run_mode) {
+            uint32_t Xc = (Pc & 0x3ffffu) | nWX;
+            uint32_t visit = (Pc >> 18) & 15u;
+            uint32_t Yc = 0;
+            bool have_y = false;
+            uint32_t newC = 0, newCs = 0, nX = 0, nY = 0;
+            if (((Pc >> 22) & 1u) && nWX == 0) {  // run mode
                 if (!cd.code(t, CTX_AGG, 0)) {
-                    ci0 = 4;
+                    visit = 0;
                 } else {
                     const uint32_t hi = cd.code(t, CTX_UNI, 0);
                     const uint32_t lo = cd.code(t, CTX_UNI, 0);
                     const uint32_t rlen = (hi << 1) | lo;
                     // a clean stripe has no significant neighbour: sign index 0
                     const uint32_t sg = dec_sign(cd, t, sc2, 0u, 0u, 0u);
-                    Xc |= 1u << (3 * rlen + 4);
-                    Yc |= sg << (3 * rlen + 4);
-                    nWX |= 1u << (3 * rlen + 3);
-                    nWY |= sg << (3 * rlen + 3);
+                    Xc |= 1u << (3u * rlen + 4u);
+                    Yc = sg << (3u * rlen + 4u);
+                    have_y = true;
+                    nX = 1u << (3u * rlen + 3u);
+                    nY = sg << (3u * rlen + 3u);
                     newC = 1u << rlen;
                     newCs = sg << rlen;
-                    ci0 = (int)rlen + 1;
+                    visit &= ~((2u << rlen) - 1u);  // rows after rlen
                 }
             }
-#pragma unroll
-            for (int ci = 0; ci < 4; ci++) {
-                if (ci >= nrows) break;
-                if (ci < ci0) continue;
-                const uint32_t nb = (Xc >> (3 * ci)) & 0x1ffu;
-                if ((nb & 0x10u) || ((visC >> ci) & 1u)) continue;
+            while (visit) {
+                const uint32_t ci = (uint32_t)__builtin_ctz(visit);
+                visit &= visit - 1u;
+                const uint32_t nb = (Xc >> (3u * ci)) & 0x1ffu;
                 const uint32_t bit = cd.code(t, nb ? zc_ctx(t, nb) : (uint32_t)CTX_ZC, 0);
                 if (bit) {
-                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, (uint32_t)ci);
-                    Xc |= 1u << (3 * ci + 4);
-                    Yc |= sg << (3 * ci + 4);
-                    nWX |= 1u << (3 * ci + 3);
-                    nWY |= sg << (3 * ci + 3);
+                    if (!have_y) {
+                        Yc = rl(Y, x) | nWY;
+                        have_y = true;
+                    }
+                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, ci);
+                    Xc |= 1u << (3u * ci + 4u);
+                    Yc |= sg << (3u * ci + 4u);
+                    nX |= 1u << (3u * ci + 3u);
+                    nY |= sg << (3u * ci + 3u);
                     newC |= 1u << ci;
                     newCs |= sg << ci;
                 }
             }
             if (newC) upd = wl(upd, newC | (newCs << 4), x);
+            nWX = nX;
+            nWY = nY;
             prevx = x;
         }
         st.sig |= (uint64_t)(upd & 15u) << (4 * s);
