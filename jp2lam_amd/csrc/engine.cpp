@@ -96,6 +96,17 @@ Engine::Engine() {
     J2L_HIP(hipMemcpy(tables_, t.data(), sizeof(uint32_t) * t.size(), hipMemcpyHostToDevice));
 }
 
+Engine::T1Tuning& Engine::t1_tuning() {
+    static T1Tuning t;
+    return t;
+}
+void Engine::clear_t1_tuning() {
+    T1Tuning& t = t1_tuning();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.chosen.clear();
+    t.probing.clear();
+}
+
 Engine::~Engine() {
     if (tables_) (void)hipFree(tables_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -285,12 +296,17 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     if (t1_mode == 3) {
         coder = (uint32_t)nb >= lane_min_blocks ? 2 : 1;
     } else if (t1_mode == 0 && (uint32_t)nb >= lane_min_blocks) {
-        auto it = tuned_.find(bucket);
-        if (it != tuned_.end()) {
+        T1Tuning& tt = t1_tuning();
+        std::lock_guard<std::mutex> g(tt.mu);
+        auto it = tt.chosen.find(bucket);
+        if (it != tt.chosen.end()) {
             coder = it->second.coder;
             frac = it->second.frac;
-        } else {
+        } else if (!tt.probing[bucket]) {
+            tt.probing[bucket] = true;
             probe = true;
+        } else {
+            coder = 4;
         }
     }
     static constexpr float kProbeFrac[2] = {0.55f, 0.75f};
@@ -612,7 +628,10 @@ t1_again:
             if (t_h0 < t) t = t_h0, best = {4, kProbeFrac[0]};
             if (t_h1 < t) t = t_h1, best = {4, kProbeFrac[1]};
         }
-        tuned_[bucket] = best;
+        T1Tuning& tt = t1_tuning();
+        std::lock_guard<std::mutex> g(tt.mu);
+        tt.chosen[bucket] = best;
+        tt.probing[bucket] = false;
     }
     last_t1_lanes = run == 2;
     last_t1_coder = run;

@@ -100,12 +100,21 @@ public:
     bool last_t1_lanes = false;
     int last_t1_coder = 1;
     // size class -> coder (1 wave, 2 lane, 4 hybrid) and the hybrid's split
-    // point; the tuner tries the hybrid at two split points
+    // point; the tuner tries the hybrid at two split points.  The table is
+    // process-wide (the fastest coder is a property of the device, and every
+    // serving slot has its own engine): a size class is probed once, and
+    // batches of a class another engine is probing run the hybrid.
     struct T1Choice {
         int coder;
         float frac;
     };
-    std::map<int, T1Choice> tuned_;
+    struct T1Tuning {
+        std::mutex mu;
+        std::map<int, T1Choice> chosen;
+        std::map<int, bool> probing;
+    };
+    static T1Tuning& t1_tuning();
+    static void clear_t1_tuning();
     // Stage timings of the last call (ms, from HIP events): prepare+DWT,
     // Tier-1, PCRD+pack, host Tier-2.
     double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;
