@@ -1297,14 +1297,21 @@ __device__ __forceinline__ void stream_mq(int b, const BlockJob* __restrict__ jo
         const uint32_t cur = nxt;
         nxt = base + 64u + lane < total_w ? words[base + 64u + lane] : 0u;
         const uint32_t cnt = min(64u, total_w - base);
-        for (uint32_t q = 0; q < cnt; q++) {
-            const uint32_t w = rl(cur, q);
-            mqe_symbol(m, w & 31u, w << 24);
-            mqe_symbol(m, (w >> 8) & 31u, w << 16);
-            mqe_symbol(m, (w >> 16) & 31u, w << 8);
-            mqe_symbol(m, (w >> 24) & 31u, w);
+        for (uint32_t q = 0; q < cnt;) {
+            if (q + 4u <= cnt && base + q + 4u <= kend) {
+                // four words inside pass k: one 16-decision group
+                mqe_group16(m, rl(cur, q), rl(cur, q + 1u), rl(cur, q + 2u), rl(cur, q + 3u));
+                q += 4u;
+            } else {
+                const uint32_t w = rl(cur, q);
+                mqe_symbol(m, w & 31u, w << 24);
+                mqe_symbol(m, (w >> 8) & 31u, w << 16);
+                mqe_symbol(m, (w >> 16) & 31u, w << 8);
+                mqe_symbol(m, (w >> 24) & 31u, w);
+                q++;
+            }
             mqe_spill(m);
-            if (base + q + 1u == kend) {  // end of pass k
+            if (base + q == kend) {  // end of pass k
                 if (k < 64u) snap0 = wlane(snap0, m.pos, k);
                 else snap1 = wlane(snap1, m.pos, k - 64u);
                 k++;
