@@ -237,12 +237,15 @@ double jp2g_last_lambda(int group);
  * every block the same way, t1.rs:287-290): mode 0 = tuned: the first batch
  * of each size class (bit length of its code-block count) from
  * `lane_min_blocks` up runs the coders below (the hybrid at split points
- * 0.55 and 0.75) and the fastest is kept for the class, smaller batches use one wavefront per code-block; 1 = one
- * wavefront per code-block; 2 = one lane per code-block; 3 = one lane per
- * code-block from `lane_min_blocks` code-blocks up; 4 = hybrid: blocks
- * whose formed symbol stream is longer than 0.55 x the batch's longest are
- * coded by a wavefront each, the rest by lanes, in one launch.  Setting
- * the mode clears the tuned choices.  Output is the same in every mode. */
+ * 0.55 and 0.75) and the fastest is kept for the class, smaller batches
+ * use one wavefront per code-block; the tuned choices are shared by every
+ * engine in the process (serving slots included), and a batch of a class
+ * that another engine is probing runs the hybrid; 1 = one wavefront per
+ * code-block; 2 = one lane per code-block; 3 = one lane per code-block
+ * from `lane_min_blocks` code-blocks up; 4 = hybrid: blocks whose formed
+ * symbol stream is longer than 0.55 x the batch's longest are coded by a
+ * wavefront each, the rest by lanes, in one launch.  Setting the mode
+ * clears the tuned choices.  Output is the same in every mode. */
 void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks);
 /* 1 when the last encode ran the lane-per-block coder. */
 int jp2g_last_t1_lanes(void);
