@@ -47,6 +47,14 @@ def parse():
     return ap.parse_args()
 
 
+_T0 = time.perf_counter()
+
+
+def note(msg):
+    """Progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def enc_plain_host(j, px, q):
     return j.encode_u8(px, q)
 
@@ -84,6 +92,7 @@ def main():
     torch.cuda.synchronize()
 
     tim = (np.zeros(4, np.float64), np.zeros(4, np.float64))
+    note(f"plain path: {W}x{H} q{Q}, {args.warmup} warm-up + {args.steps} timed steps")
 
     def step():
         enc = j.encode_many([((d_in.data_ptr(), H, W, 1), Q)], device=True)[0]
@@ -209,6 +218,7 @@ def main():
         pool = [synth.document_page(W, H, seed=2 + 1000 * k) for k in range(4)]
         want = [enc_plain_host(j, p, Q) for p in pool]
         n = max(1, int(args.serve_rate * args.serve_seconds))
+        note(f"serving: {n} Poisson arrivals at {args.serve_rate:g}/s")
         rng = np.random.default_rng(5)
         arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
         ids = [None] * n
@@ -231,6 +241,8 @@ def main():
                 if d > 0:
                     time.sleep(d)
                 ids[i] = srv.submit_encode(pool[i % len(pool)], Q)
+                if (i + 1) % 1000 == 0:
+                    note(f"serving: {i + 1}/{n} submitted")
             th.join()
             t_end = time.perf_counter()
             st = srv.stats()
@@ -329,7 +341,9 @@ def main():
         # mixed-size queue: 70 % documents / 30 % photos, q in {50, 75, 85, 100},
         # encoded and decoded in device batches of --mixed-batch pages
         if args.mixed_pages > 0:
+            note(f"mixed queue: generating {args.mixed_pages} pages")
             host = list(synth.mixed_pages(args.mixed_pages, seed=4))
+            note("mixed queue: encode and decode in device batches")
             devp = [torch.from_numpy(p).to(dev) for p, _ in host]
             items = [((d.data_ptr(), p.shape[0], p.shape[1], 1 if p.ndim == 2 else 3), q)
                      for d, (p, q) in zip(devp, host)]
@@ -343,20 +357,36 @@ def main():
             streams = [None] * len(items)
 
             def enc_all():
-                for ch in chunks:
+                for c, ch in enumerate(chunks):
+                    if c and c % 4 == 0:
+                        note(f"mixed queue: encoded {c}/{len(chunks)} batches")
                     for k, r in zip(ch, j.encode_many([items[k] for k in ch], device=True)):
                         if isinstance(r, Exception):
                             raise r
                         streams[k] = r
 
             def dec_all():
-                for ch in dchunks:
+                for c, ch in enumerate(dchunks):
+                    if c and c % 4 == 0:
+                        note(f"mixed queue: decoded {c}/{len(dchunks)} batches")
                     for r in j.decode_many([streams[k] for k in ch],
                                            device_outputs=[(outs[k].data_ptr(), outs[k].numel()) for k in ch]):
                         if isinstance(r, Exception):
                             raise r
 
             _, em = timed(enc_all, 1)
+            # streams the reference decoder rejects, as ours does (its
+            # zero-length first contributions, decode/t2.rs:243), stay out of
+            # the decode half and the serving leg's decode requests, and are
+            # listed with the reason
+            rejected = {}
+            for k in dec_ids:
+                try:
+                    j.decode_u8(streams[k])
+                except j.DecodeFailed as e:
+                    rejected[k] = str(e)
+            dec_ids = [k for k in dec_ids if k not in rejected]
+            dchunks = [dec_ids[i:i + B] for i in range(0, len(dec_ids), B)]
             _, dm = timed(dec_all, 1)
             qmpx = sum(p.shape[0] * p.shape[1] for p, _ in host) / 1e6
             dmpx = sum(host[k][0].shape[0] * host[k][0].shape[1] for k in dec_ids) / 1e6
@@ -369,7 +399,8 @@ def main():
             configs["mixed_queue"] = {
                 "pages": len(host), "megapixels": round(qmpx, 1), "batch_pages": B,
                 "gray_pages": sum(1 for p, _ in host if p.ndim == 2),
-                "decoded_pages": len(dec_ids), "encode_s": round(em / 1e3, 3), "decode_s": round(dm / 1e3, 3),
+                "decoded_pages": len(dec_ids), "decode_rejected_as_by_reference": {str(k): v for k, v in rejected.items()},
+                "encode_s": round(em / 1e3, 3), "decode_s": round(dm / 1e3, 3),
                 "encode_mpx_s": round(qmpx / (em / 1e3), 1), "decode_mpx_s": round(dmpx / (dm / 1e3), 1),
                 "batched_equals_single": bool(same), "parity_vs_oracle_smallest_page": bool(ok)}
             del devp, outs
@@ -411,12 +442,16 @@ def main():
                             k = pick[i]
                             rids[i] = (srv.submit_encode(host[k][0], host[k][1]) if is_enc[i]
                                        else srv.submit_decode(streams[k]))
+                            if (i + 1) % 1000 == 0:
+                                note(f"mixed serving: {i + 1}/{nreq} submitted")
                         th.join()
                         return rdone, t0m, time.perf_counter()
 
                     # the first pass warms the arenas, the pinned pool and the
                     # coder tuner's size classes; the second is reported
+                    note(f"mixed serving: {nreq} arrivals, warm-up pass")
                     stream_once()
+                    note("mixed serving: measured pass")
                     rdone, t0m, t_endm = stream_once()
                 okm = all((rdone[i].value == want_enc[pick[i]]) if is_enc[i]
                           else np.array_equal(rdone[i].value, want_dec[pick[i]]) for i in range(nreq))

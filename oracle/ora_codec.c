@@ -154,7 +154,7 @@ int ora_encode(const int32_t *const *planes, uint32_t w, uint32_t h, int ncomp, 
                 refs[nb].nblocks = cx * cy;
                 refs[nb].cx = cx;
                 refs[nb].cy = cy;
-                refs[nb].blocks = malloc(sizeof(ora_block *) * (size_t)(cx * cy ? cx * cy : 1));
+                refs[nb].blocks = malloc(sizeof(ora_block *) * (size_t)(cx * cy > 0 ? cx * cy : 1));
                 for (int i = 0; i < cx * cy; i++) refs[nb].blocks[i] = &enc->blocks[bstart + i];
                 bstart += cx * cy;
                 nb++;

@@ -580,6 +580,9 @@ int ora_parse_packets(const ora_cs_header *hd, ora_dec_block *blocks, int nblock
     size_t plen = hd->payload_len, pos = 0;
     tagtree *incl = calloc((size_t)nbands, sizeof(tagtree));
     tagtree *imsb = calloc((size_t)nbands, sizeof(tagtree));
+    /* a packet's contributions in header order (decode/t2.rs:96-152 collects
+       them in a Vec): at most one per code-block */
+    int *order = malloc((size_t)(nblocks > 0 ? nblocks : 1) * sizeof(int));
     for (int i = 0; i < nbands; i++) {
         tgt_init(&incl[i], bands[i].cx > 0 ? bands[i].cx : 1, bands[i].cy > 0 ? bands[i].cy : 1, 0x7fffffff);
         tgt_init(&imsb[i], bands[i].cx > 0 ? bands[i].cx : 1, bands[i].cy > 0 ? bands[i].cy : 1, 0x7fffffff);
@@ -591,7 +594,6 @@ int ora_parse_packets(const ora_cs_header *hd, ora_dec_block *blocks, int nblock
             uint32_t present;
             size_t body_total = 0;
             int contrib_first = -1, contrib_count = 0;
-            int order[4096];
             if (bior_bit(&br, &present)) { rc = ORA_DECODE_FAILED; snprintf(err, 256, "packet header ended before requested bit"); break; }
             if (present) {
                 for (int bi = 0; bi < nbands && !rc; bi++) {
@@ -645,7 +647,7 @@ int ora_parse_packets(const ora_cs_header *hd, ora_dec_block *blocks, int nblock
                         blk->included = 1;
                         blk->passes = (int)np;
                         blk->len = len;
-                        if (contrib_count < 4096) order[contrib_count] = bands[bi].first + k;
+                        order[contrib_count] = bands[bi].first + k;
                         if (contrib_first < 0) contrib_first = bands[bi].first + k;
                         contrib_count++;
                         body_total += len;
@@ -655,11 +657,10 @@ int ora_parse_packets(const ora_cs_header *hd, ora_dec_block *blocks, int nblock
             if (rc) break;
             pos += br.pos;
             if (pos + body_total > plen) { rc = ORA_DECODE_FAILED; snprintf(err, 256, "packet body extends past tile payload"); break; }
-            for (int i = 0; i < contrib_count && i < 4096; i++) {
+            for (int i = 0; i < contrib_count; i++) {
                 blocks[order[i]].data = pay + pos;
                 pos += blocks[order[i]].len;
             }
-            if (contrib_count > 4096) { rc = ORA_DECODE_FAILED; snprintf(err, 256, "too many contributions in one packet"); break; }
             continue;
         trunc:
             rc = ORA_DECODE_FAILED;
@@ -676,5 +677,6 @@ int ora_parse_packets(const ora_cs_header *hd, ora_dec_block *blocks, int nblock
     }
     free(incl);
     free(imsb);
+    free(order);
     return rc;
 }

@@ -258,6 +258,19 @@ def test_full_page_document_q85(gpu, ora):
     assert np.array_equal(j.decode_u8(ours), ora.decode(ref))
 
 
+def test_packet_with_more_than_4096_contributions(gpu, ora):
+    """A 4736x4736 lossless page: the finest resolution's packet carries
+    3 x 37 x 37 = 4107 code-block contributions (the reference collects them
+    in a Vec, decode/t2.rs:96-152).  Codestream identical to the oracle's,
+    decode lossless and identical to the oracle's decoder."""
+    px = synth.document_page(4736, 4736, seed=9)
+    ours = j.encode_u8(px, 100)
+    assert ours == ora.encode(px, 100)
+    out = j.decode_u8(ours)
+    assert np.array_equal(out, px)
+    assert np.array_equal(out, ora.decode(ours))
+
+
 def test_mixed_batch_equals_single(gpu, ora):
     pages = list(synth.mixed_pages(6, scale=0.08))
     singles = [j.encode_u8(px, q) for px, q in pages]
