@@ -286,10 +286,12 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     // wave-per-block kernel below.
     // Tier-1 coder for this batch: forced (1 wave / 2 lane per block), a
     // fixed threshold (3), or tuned (0): the first batch of each size class
-    // (bit length of the block count) from lane_min_blocks up runs both
-    // coders on the same coefficients and keeps the faster; smaller batches
-    // use the wave coder.
-    const int bucket = nb ? 32 - __builtin_clz((unsigned)nb) : 0;
+    // (bit lengths of the block count and of the image count: one large
+    // image and several pages of the same block count favour different
+    // coders) from lane_min_blocks up runs every coder on the same
+    // coefficients and keeps the fastest; smaller batches use the wave coder.
+    const int bucket = (nb ? 32 - __builtin_clz((unsigned)nb) : 0) * 8 +
+                       std::min(7, 32 - __builtin_clz((unsigned)imgs.size()));
     int coder = t1_mode == 1 || t1_mode == 2 || t1_mode == 4 ? t1_mode : 1;
     float frac = hybrid_frac;
     bool probe = false;

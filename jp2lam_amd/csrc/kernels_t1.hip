@@ -611,16 +611,17 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         : [d] "s"(d31), [tab] "v"(m.tab), [f16] "s"(0x60010u)
         : "scc");
 }
-// Coder registers at the start of a block (mq/mod.rs:183-200).  The scalar
-// zeros share one SGPR definition: a constant 0 shared with the ring (a VGPR
-// starting at 0) can be materialised in a VGPR and reach the asm operands
-// through the pass loop's phis, which the backend cannot turn into SGPRs.
+// Coder registers at the start of a block (mq/mod.rs:183-200).  Their
+// initial values come from one asm that writes SGPRs: a constant the
+// compiler materialises with v_mov (0 is shared with the ring, a VGPR) would
+// reach the asm operands through the pass loop's phis as a VGPR, which the
+// backend cannot turn back into an SGPR.
 __device__ __forceinline__ void mqe_init(MqE& m, uint32_t mq, uint8_t* out, uint32_t cap) {
-    uint32_t z;
-    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
-    m.a = 0x8000u;
+    uint32_t z, a0, ct0;
+    asm volatile("s_mov_b32 %0, 0\n\ts_mov_b32 %1, 0x8000\n\ts_mov_b32 %2, 12" : "=s"(z), "=s"(a0), "=s"(ct0));
+    m.a = a0;
     m.c = z;
-    m.ct = 12;
+    m.ct = ct0;
     m.pos = z;
     m.last = z;
     m.wbase = z;
@@ -1298,19 +1299,21 @@ __device__ __forceinline__ void stream_mq(int b, const BlockJob* __restrict__ jo
         nxt = base + 64u + lane < total_w ? words[base + 64u + lane] : 0u;
         const uint32_t cnt = min(64u, total_w - base);
         for (uint32_t q = 0; q < cnt;) {
-            if (q + 4u <= cnt && base + q + 4u <= kend) {
-                // four words inside pass k: one 16-decision group
+            // words up to the end of pass k or of the chunk: groups of four,
+            // then single words
+            const uint32_t lim = min(cnt, kend - base);
+            for (; q + 4u <= lim; q += 4u) {
                 mqe_group16(m, rl(cur, q), rl(cur, q + 1u), rl(cur, q + 2u), rl(cur, q + 3u));
-                q += 4u;
-            } else {
+                mqe_spill(m);
+            }
+            for (; q < lim; q++) {
                 const uint32_t w = rl(cur, q);
                 mqe_symbol(m, w & 31u, w << 24);
                 mqe_symbol(m, (w >> 8) & 31u, w << 16);
                 mqe_symbol(m, (w >> 16) & 31u, w << 8);
                 mqe_symbol(m, (w >> 24) & 31u, w);
-                q++;
+                mqe_spill(m);
             }
-            mqe_spill(m);
             if (base + q == kend) {  // end of pass k
                 if (k < 64u) snap0 = wlane(snap0, m.pos, k);
                 else snap1 = wlane(snap1, m.pos, k - 64u);
