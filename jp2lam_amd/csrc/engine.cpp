@@ -6,6 +6,8 @@
 #include <functional>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -286,12 +288,11 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     // wave-per-block kernel below.
     // Tier-1 coder for this batch: forced (1 wave / 2 lane per block), a
     // fixed threshold (3), or tuned (0): the first batch of each size class
-    // (bit lengths of the block count and of the image count: one large
-    // image and several pages of the same block count favour different
-    // coders) from lane_min_blocks up runs every coder on the same
-    // coefficients and keeps the fastest; smaller batches use the wave coder.
-    const int bucket = (nb ? 32 - __builtin_clz((unsigned)nb) : 0) * 8 +
-                       std::min(7, 32 - __builtin_clz((unsigned)imgs.size()));
+    // (bit length of the block count) from lane_min_blocks up runs every
+    // coder on the same coefficients and keeps the fastest; smaller batches
+    // use the wave coder.  (Classes keyed on the image count as well probed
+    // far more often in serving, where each probe stalls a batch.)
+    const int bucket = nb ? 32 - __builtin_clz((unsigned)nb) : 0;
     int coder = t1_mode == 1 || t1_mode == 2 || t1_mode == 4 ? t1_mode : 1;
     float frac = hybrid_frac;
     bool probe = false;
@@ -690,35 +691,30 @@ t1_again:
     // ---- host Tier-2 + container --------------------------------------------------------
     auto th0 = std::chrono::steady_clock::now();
     last_blocks_.clear();
-    size_t pk = 0;
-    for (size_t k = 0; k < imgs.size(); k++) {
-        Img& im = imgs[k];
-        EncOut& out = outs[im.idx];
-        const Plan& p = im.plan;
-        const uint8_t* body = packed_host + pk;
-        pk += totv[k];
-        bool overflow = false;
-        for (uint32_t b = im.blk_first; b < im.blk_first + im.blk_count; b++) overflow |= resv[b].overflow != 0;
-        if (overflow) {
-            out.status = ENCODE_FAILED;
-            out.message = "code-block segment exceeded its device slot";
-            continue;
+    std::vector<size_t> pkoff(imgs.size());
+    {
+        size_t pk = 0;
+        for (size_t k = 0; k < imgs.size(); k++) {
+            pkoff[k] = pk;
+            pk += totv[k];
         }
-        std::vector<CodedBlock> cbs(im.blk_count);
-        for (uint32_t b = 0; b < im.blk_count; b++) {
-            const Selection& s = selv[im.blk_first + b];
-            cbs[b] = {s.keep, s.zbp, s.body_len};
-        }
-        if (keep_debug) {
+    }
+    if (keep_debug) {
+        for (size_t k = 0; k < imgs.size(); k++) {
+            const Img& im = imgs[k];
+            bool overflow = false;
+            for (uint32_t b = im.blk_first; b < im.blk_first + im.blk_count; b++) overflow |= resv[b].overflow != 0;
+            if (overflow) continue;  // the image fails below; no statistics
             for (uint32_t b = 0; b < im.blk_count; b++) {
                 const BlockJob& j = jobs[im.blk_first + b];
                 const BlockResult& r = resv[im.blk_first + b];
+                const Selection& sl = selv[im.blk_first + b];
                 BlockStat bs{};
                 bs.mbp = r.mbp;
                 bs.zbp = r.zbp;
                 bs.cls = r.cls;
                 bs.npasses = r.npasses;
-                bs.keep = cbs[b].keep;
+                bs.keep = sl.keep;
                 bs.band = j.band;
                 bs.comp = blk_comp[im.blk_first + b];
                 bs.res = blk_res[im.blk_first + b];
@@ -727,15 +723,42 @@ t1_again:
                 bs.x1 = (uint16_t)(j.x0 + j.w);
                 bs.y1 = (uint16_t)(j.y0 + j.h);
                 bs.nbytes = r.nbytes;
-                bs.body_len = cbs[b].seg_len;
+                bs.body_len = sl.body_len;
                 bs.max_mag = r.max_mag;
                 bs.nnz = r.nnz;
                 last_blocks_.push_back(bs);
             }
         }
+    }
+    // Tier-2 and containers of one image, written once into its output: the
+    // packet headers go to a side buffer first, which fixes the codestream
+    // length the JP2 header needs, then header, packets and bodies are
+    // appended in order.
+    auto assemble = [&](size_t k) {
+        Img& im = imgs[k];
+        EncOut& out = outs[im.idx];
+        const Plan& p = im.plan;
+        const uint8_t* body = packed_host + pkoff[k];
+        bool overflow = false;
+        for (uint32_t b = im.blk_first; b < im.blk_first + im.blk_count; b++) overflow |= resv[b].overflow != 0;
+        if (overflow) {
+            out.status = ENCODE_FAILED;
+            out.message = "code-block segment exceeded its device slot";
+            return;
+        }
+        std::vector<CodedBlock> cbs(im.blk_count);
+        for (uint32_t b = 0; b < im.blk_count; b++) {
+            const Selection& sl = selv[im.blk_first + b];
+            cbs[b] = {sl.keep, sl.zbp, sl.body_len};
+        }
         // packets in LRCP order: resolution, component (t2.rs:409-470)
-        std::vector<uint8_t> payload;
-        payload.reserve(totv[k] + 64 * (p.levels + 1) * p.ncomp + 1024);
+        struct Pkt {
+            size_t h0, h1, b0, blen;
+        };
+        std::vector<Pkt> pkts;
+        pkts.reserve((size_t)(p.levels + 1) * p.ncomp);
+        std::vector<uint8_t> hdrs;
+        hdrs.reserve(64 * (size_t)(p.levels + 1) * p.ncomp + 1024);
         uint32_t bi = 0;
         size_t bo = 0;
         for (int r = 0; r <= p.levels; r++) {
@@ -750,25 +773,27 @@ t1_again:
                     bands[nbands++] = {cx, cy, cbs.data() + bi};
                     bi += (uint32_t)(cx * cy);
                 }
-                packet_header(bands, nbands, payload);
+                const size_t h0 = hdrs.size();
+                packet_header(bands, nbands, hdrs);
                 size_t blen = 0;
                 for (uint32_t b = first; b < bi; b++) blen += cbs[b].seg_len;
-                payload.insert(payload.end(), body + bo, body + bo + blen);
+                pkts.push_back({h0, hdrs.size(), bo, blen});
                 bo += blen;
             }
         }
-        std::vector<uint8_t> cs;
-        cs.reserve(payload.size() + 256);
-        write_codestream_head(p, payload.size(), cs);
-        cs.insert(cs.end(), payload.begin(), payload.end());
-        write_codestream_tail(cs);
-        if (p.format == 0) {
-            out.bytes.reserve(cs.size() + 96);
-            write_jp2_head(p, cs.size(), out.bytes);
-            out.bytes.insert(out.bytes.end(), cs.begin(), cs.end());
-        } else {
-            out.bytes = std::move(cs);
+        std::vector<uint8_t> head;
+        write_codestream_head(p, hdrs.size() + bo, head);
+        const size_t cs_len = head.size() + hdrs.size() + bo + 2;  // + EOC
+        std::vector<uint8_t>& o = out.bytes;
+        o.clear();
+        o.reserve(cs_len + (p.format == 0 ? 128 : 0));
+        if (p.format == 0) write_jp2_head(p, cs_len, o);
+        o.insert(o.end(), head.begin(), head.end());
+        for (const Pkt& q : pkts) {
+            o.insert(o.end(), hdrs.begin() + (ptrdiff_t)q.h0, hdrs.begin() + (ptrdiff_t)q.h1);
+            o.insert(o.end(), body + q.b0, body + q.b0 + q.blen);
         }
+        write_codestream_tail(o);
         if (items[im.idx].metrics) {
             if (im.mseg < 0) {
                 // backend.rs:308-314: lossless encodes
@@ -783,6 +808,33 @@ t1_again:
                 out.ssim = full ? m_out[2 * im.mseg + 1] / (double)full : 1.0;
             }
         }
+    };
+    // Images are independent: a batch of several spreads them over host
+    // threads (each writes only its own output).
+    const size_t nthr = imgs.size() < 4 ? 1 : std::min<size_t>({imgs.size(), 8, std::max(1u, std::thread::hardware_concurrency())});
+    if (nthr <= 1) {
+        for (size_t k = 0; k < imgs.size(); k++) assemble(k);
+    } else {
+        std::atomic<size_t> next{0};
+        std::vector<std::string> errs(imgs.size());
+        auto work = [&] {
+            for (size_t k; (k = next.fetch_add(1)) < imgs.size();) {
+                try {
+                    assemble(k);
+                } catch (const std::exception& e) {
+                    errs[k] = e.what();
+                }
+            }
+        };
+        std::vector<std::thread> ts;
+        for (size_t t = 1; t < nthr; t++) ts.emplace_back(work);
+        work();
+        for (auto& t : ts) t.join();
+        for (size_t k = 0; k < imgs.size(); k++)
+            if (!errs[k].empty()) {
+                outs[imgs[k].idx].status = ENCODE_FAILED;
+                outs[imgs[k].idx].message = errs[k];
+            }
     }
     t_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     return outs;
