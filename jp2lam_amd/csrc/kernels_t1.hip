@@ -711,32 +711,33 @@ __device__ __forceinline__ void mqe_spill(MqE& m) {
         m.wbase += 32u;
     }
 }
+// Code the first cnt (<= 256) symbol bytes held in the wave's lanes (lane i:
+// bytes 4i..4i+3).
+__device__ __forceinline__ void mqe_chunk(MqE& m, uint32_t sv, uint32_t cnt) {
+    uint32_t q = 0;
+    // 16 symbols move at most 32 bytes out, so one spill check per 16
+    // keeps the ring from reusing an unstored byte (at most 64 pending)
+    for (; q + 16u <= cnt; q += 16u) {
+        mqe_group16(m, rl(sv, q >> 2), rl(sv, (q >> 2) + 1u), rl(sv, (q >> 2) + 2u), rl(sv, (q >> 2) + 3u));
+        mqe_spill(m);
+    }
+    for (; q + 4u <= cnt; q += 4u) {
+        const uint32_t w = rl(sv, q >> 2);
+        mqe_symbol(m, w & 31u, w << 24);
+        mqe_symbol(m, (w >> 8) & 31u, w << 16);
+        mqe_symbol(m, (w >> 16) & 31u, w << 8);
+        mqe_symbol(m, (w >> 24) & 31u, w);
+        mqe_spill(m);
+    }
+    if (q < cnt) {
+        uint32_t w = rl(sv, q >> 2);
+        for (; q < cnt; q++, w >>= 8) mqe_symbol(m, w & 31u, w << 24);
+        mqe_spill(m);
+    }
+}
 // Code the n symbol bytes of one pass from LDS.
 __device__ __forceinline__ void mqe_pass(MqE& m, const uint32_t* symw, uint32_t n) {
-    for (uint32_t base = 0; base < n; base += 256u) {
-        const uint32_t sv = symw[(base >> 2) + lid()];
-        const uint32_t cnt = min(256u, n - base);
-        uint32_t q = 0;
-        // 16 symbols move at most 32 bytes out, so one spill check per 16
-        // keeps the ring from reusing an unstored byte (at most 64 pending)
-        for (; q + 16u <= cnt; q += 16u) {
-            mqe_group16(m, rl(sv, q >> 2), rl(sv, (q >> 2) + 1u), rl(sv, (q >> 2) + 2u), rl(sv, (q >> 2) + 3u));
-            mqe_spill(m);
-        }
-        for (; q + 4u <= cnt; q += 4u) {
-            const uint32_t w = rl(sv, q >> 2);
-            mqe_symbol(m, w & 31u, w << 24);
-            mqe_symbol(m, (w >> 8) & 31u, w << 16);
-            mqe_symbol(m, (w >> 16) & 31u, w << 8);
-            mqe_symbol(m, (w >> 24) & 31u, w);
-            mqe_spill(m);
-        }
-        if (q < cnt) {
-            uint32_t w = rl(sv, q >> 2);
-            for (; q < cnt; q++, w >>= 8) mqe_symbol(m, w & 31u, w << 24);
-            mqe_spill(m);
-        }
-    }
+    for (uint32_t base = 0; base < n; base += 256u) mqe_chunk(m, symw[(base >> 2) + lid()], min(256u, n - base));
 }
 // Byte-out for the flush (same as the asm sequence above).
 __device__ __forceinline__ void mqe_byteout(MqE& m) {
@@ -887,8 +888,8 @@ __global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t* __restr
 // The block this wave codes.  The n waves of the launch take n distinct
 // positions of the order (heavy ones from the front, light ones from the
 // back), wherever they are placed, so every block is coded exactly once.
-__device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched) {
-    uint32_t b = 0;
+__device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched, uint32_t& pos) {
+    uint32_t b = 0, at = 0;
     if (lid() == 0) {
         uint32_t hw, xcc;
         asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)\n\ts_getreg_b32 %1, hwreg(HW_REG_XCC_ID)" : "=s"(hw), "=s"(xcc));
@@ -897,9 +898,112 @@ __device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched) {
         uint32_t* ctrl = sched + 2 * (size_t)n;
         const bool first = atomicAdd(&ctrl[2 + simd], 1u) == 0u;
         const uint32_t k = atomicAdd(&ctrl[first ? 0 : 1], 1u);
-        b = sched[n + (first ? k : n - 1u - k)];
+        at = first ? k : n - 1u - k;
+        b = sched[n + at];
     }
+    pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+}
+
+// ---- heaviest blocks formed ahead, one wave per bit-plane --------------------
+// A wave codes its block's bit-planes one after another, forming each pass
+// (lane-parallel) and then coding its decisions (scalar), so the heaviest
+// block of a launch, which sets the launch's length, also pays for its own
+// formation.  The coder state at the start of plane p depends on the
+// magnitudes alone: significant = magnitude >= 2^(p+1), refined =
+// magnitude >= 2^(p+2), nothing visited.  So t1_preform_kernel forms the
+// planes of the kPre heaviest blocks (order positions 0..kPre-1 with at least
+// half the cost of position 0) in parallel, one wave per plane, and the
+// coder wave that takes such a position only codes.
+//
+// Per position a slot after the hand-out counters in sched: pass k's start
+// (in words from the slot's data) at [k], its symbol count | new
+// significance << 16 at [kMaxPasses + k], then kPrePlanes plane regions of
+// kPlaneWords.  In a plane region the passes follow each other, each
+// rounded up to 64 bytes (the coder's read unit); a plane has at most 10
+// symbols per stripe-column, 10240 in all.
+constexpr uint32_t kPre = 8;
+constexpr uint32_t kPrePlanes = 32;
+constexpr uint32_t kPlaneWords = (kSymCap + 3u * 64u) / 4u;
+constexpr uint32_t kPreSlotWords = 2u * kMaxPasses + kPrePlanes * kPlaneWords;
+__host__ __device__ __forceinline__ size_t pre_base(uint32_t n) {
+    return (2 * (size_t)n + 2 + kSimdSlots + 63) & ~(size_t)63;
+}
+__device__ __forceinline__ bool pre_formed(const uint32_t* sched, uint32_t n, uint32_t pos) {
+    if (pos >= kPre || pos >= n) return false;
+    const uint32_t c0 = sched[sched[n]], c = sched[sched[n + pos]];
+    return c > 0u && 2u * c >= c0;
+}
+
+// One wave per (position, plane): the plane's passes of a pre-formed
+// position, in the coder's pass order (plane mbp-1: cleanup only, pass 0;
+// plane p below: passes 1 + 3 (mbp - 2 - p) + 0, 1, 2).
+__global__ __launch_bounds__(64) void t1_preform_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+                                                        uint32_t n, uint32_t* __restrict__ sched) {
+    __shared__ uint32_t symw[kSymCap / 4];
+    uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
+    const uint32_t lane = lid();
+    const uint32_t slot = blockIdx.x / kPrePlanes, p = blockIdx.x % kPrePlanes;
+    if (!pre_formed(sched, n, slot)) return;
+    const BlockJob J = jobs[sched[n + slot]];
+    const int w = J.w, h = J.h;
+    uint32_t mag[64];
+    uint64_t neg;
+    BlockResult R;
+    const int mbp = load_block(J, mag, neg, R);
+    if ((int)p >= mbp) return;
+    Tables t;
+    load_tables(t, tabs, J.band);
+    uint64_t bits = 0, sig = 0, ref = 0;
+#pragma unroll
+    for (int y = 0; y < 64; y++) {
+        const uint64_t v = mag[y];
+        bits |= ((v >> p) & 1u) << y;
+        sig |= (uint64_t)((v >> (p + 1u)) != 0u) << y;
+        ref |= (uint64_t)((v >> (p + 2u)) != 0u) << y;
+    }
+    Col st = {sig, sig & neg, 0, ref};
+    uint32_t* slotp = sched + pre_base(n) + (size_t)slot * kPreSlotWords;
+    uint32_t* data = slotp + 2u * kMaxPasses + p * kPlaneWords;
+    const uint32_t top = (uint32_t)mbp - 1u;
+    uint32_t k = p == top ? 0u : 1u + 3u * (top - 1u - p), off = 0;
+    auto put = [&](uint32_t cnt, uint32_t ns) {
+        __syncthreads();
+        for (uint32_t i = lane; i < (cnt + 3u) >> 2; i += 64u) data[off + i] = symw[i];
+        if (lane == 0) {
+            slotp[k] = p * kPlaneWords + off;
+            slotp[kMaxPasses + k] = cnt | ns << 16;
+        }
+        off += ((cnt + 63u) >> 6) << 4;
+        k++;
+        __syncthreads();
+    };
+    uint32_t ns, cnt;
+    if (p != top) {
+        cnt = form_spp(t, st, bits, neg, w, h, symb, ns);
+        put(cnt, ns);
+        cnt = form_mrp(st, bits, w, h, symb);
+        put(cnt, 0);
+    }
+    cnt = form_cup(t, st, bits, neg, w, h, symb, ns);
+    put(cnt, ns);
+}
+
+// 64 symbol bytes, read with scalar loads (the read unit of pre-formed
+// passes; 64-byte aligned).  The region is written by an earlier launch only,
+// so it is read through the constant address space: the coder's asm
+// statements would otherwise count as possible stores to it and turn the
+// loads into vector loads.
+struct W16 {
+    uint32_t w[16];
+};
+typedef const __attribute__((address_space(4))) uint32_t const_u32;
+__device__ __forceinline__ W16 load16(const uint32_t* p) {
+    const const_u32* q = (const const_u32*)p;
+    W16 r;
+#pragma unroll
+    for (int i = 0; i < 16; i++) r.w[i] = q[i];
+    return r;
 }
 
 // Tier-1 encoder: one wavefront per code-block (the block from
@@ -909,11 +1013,12 @@ __device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched) {
 __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                        uint8_t* __restrict__ out, BlockResult* __restrict__ results,
                                                        uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs,
-                                                       uint32_t* __restrict__ sched) {
+                                                       uint32_t* __restrict__ sched, const uint32_t* __restrict__ pre) {
     __shared__ uint32_t symw[kSymCap / 4];
     uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
     const int lane = lid();
-    const uint32_t blk = t1_pick_block(gridDim.x, sched);
+    uint32_t pos;
+    const uint32_t blk = t1_pick_block(gridDim.x, sched, pos);
     const BlockJob J = jobs[blk];
     const int w = J.w, h = J.h;
     uint32_t mag[64];
@@ -947,6 +1052,55 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
         np++;
     };
     const int top = mbp - 1;
+    if (pre_formed(sched, gridDim.x, pos)) {
+        // Code the pre-formed passes: 64 symbols per step from SGPRs, the
+        // next 64 (or the next pass's first) loaded while these are coded.
+        const uint32_t* slotp = pre + (size_t)pos * kPreSlotWords;
+        const uint32_t* data = slotp + 2u * kMaxPasses;
+        const uint32_t hs0 = slotp[lane], hn0 = slotp[kMaxPasses + lane];
+        const uint32_t hs1 = lane < kMaxPasses - 64 ? slotp[64 + lane] : 0u;
+        const uint32_t hn1 = lane < kMaxPasses - 64 ? slotp[kMaxPasses + 64 + lane] : 0u;
+        const uint32_t npass = 3u * (uint32_t)mbp - 2u;
+        uint32_t at = rl(hs0, 0);
+        W16 cur = load16(data + at);
+        for (uint32_t k = 0; k < npass; k++) {
+            const uint32_t S = k < 64u ? rl(hs0, k) : rl(hs1, k - 64u);
+            const uint32_t nn = k < 64u ? rl(hn0, k) : rl(hn1, k - 64u);
+            const uint32_t cnt = nn & 0xffffu;
+            if (S != at) {
+                at = S;
+                cur = load16(data + at);
+            }
+            const uint32_t k1 = k + 1u;
+            const uint32_t after = k1 >= npass ? at : k1 < 64u ? rl(hs0, k1) : rl(hs1, k1 - 64u);
+            uint32_t b = 0;
+            for (; b + 64u <= cnt; b += 64u) {
+                const uint32_t nxt = b + 64u < cnt ? at + 16u : after;
+                const W16 nx = load16(data + nxt);
+                mqe_group16(m, cur.w[0], cur.w[1], cur.w[2], cur.w[3]);
+                mqe_spill(m);
+                mqe_group16(m, cur.w[4], cur.w[5], cur.w[6], cur.w[7]);
+                mqe_spill(m);
+                mqe_group16(m, cur.w[8], cur.w[9], cur.w[10], cur.w[11]);
+                mqe_spill(m);
+                mqe_group16(m, cur.w[12], cur.w[13], cur.w[14], cur.w[15]);
+                mqe_spill(m);
+                cur = nx;
+                at = nxt;
+            }
+            if (b < cnt) {
+                // last partial 64: into a VGPR for the generic chunk coder
+                const W16 nx = load16(data + after);
+                uint32_t sv = 0;
+#pragma unroll
+                for (int i = 0; i < 16; i++) sv = wlane(sv, cur.w[i], (uint32_t)i);
+                mqe_chunk(m, sv, cnt - b);
+                cur = nx;
+                at = after;
+            }
+            record(nn >> 16);
+        }
+    } else
     for (int p = top; p >= 0; p--) {
         uint64_t bits = 0;
 #pragma unroll
@@ -1742,13 +1896,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
     }
 }
 
-size_t t1_sched_words(int n) { return 2 * (size_t)n + 2 + kSimdSlots; }
+size_t t1_sched_words(int n) { return pre_base((uint32_t)n) + kPre * (size_t)kPreSlotWords; }
 void launch_t1_encode(const BlockJob* jobs, int n, const uint32_t* tabs, uint8_t* out, BlockResult* res, uint32_t* snaps,
                       uint16_t* newsigs, uint32_t* sched, hipStream_t st) {
     if (n <= 0) return;
     t1_cost_kernel<<<n, 64, 0, st>>>(jobs, sched);
     t1_order_kernel<<<1, 1024, 0, st>>>(n, sched);
-    t1_encode_kernel<<<n, 64, 0, st>>>(jobs, tabs, out, res, snaps, newsigs, sched);
+    t1_preform_kernel<<<kPre * kPrePlanes, 64, 0, st>>>(jobs, tabs, (uint32_t)n, sched);
+    t1_encode_kernel<<<n, 64, 0, st>>>(jobs, tabs, out, res, snaps, newsigs, sched, sched + pre_base((uint32_t)n));
 }
 void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResult* res, uint32_t* pend, uint16_t* newsigs,
                     uint32_t* soff16, uint8_t* pool, unsigned long long* pool_used, unsigned long long pool_cap,
