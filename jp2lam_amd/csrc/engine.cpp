@@ -28,6 +28,28 @@ struct Layout {
     }
 };
 
+// A table bound for device offset dst, staged at hoff in the page-locked
+// arena.
+struct Staged {
+    size_t dst, hoff, n;
+    const void* src;
+};
+// Fill the arena and upload: consecutive tables whose device and host
+// offsets advance alike go in one copy (the gaps between them are
+// alignment padding on both sides).
+void copy_staged(const std::vector<Staged>& staged, char* D, char* P, hipStream_t st) {
+    for (size_t i = 0; i < staged.size();) {
+        size_t j = i + 1;
+        while (j < staged.size() && staged[j].dst >= staged[i].dst &&
+               staged[j].dst - staged[i].dst == staged[j].hoff - staged[i].hoff)
+            j++;
+        for (size_t k = i; k < j; k++) std::memcpy(P + staged[k].hoff, staged[k].src, staged[k].n);
+        const size_t n = staged[j - 1].hoff + staged[j - 1].n - staged[i].hoff;
+        J2L_HIP(hipMemcpyAsync(D + staged[i].dst, P + staged[i].hoff, n, hipMemcpyHostToDevice, st));
+        i = j;
+    }
+}
+
 struct Events {
     hipEvent_t e[8];
     Events() {
@@ -264,23 +286,18 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     }
     if (slot_total > 0xffffffffull) throw Error(ENCODE_FAILED, "batch too large for one launch");
 
-    const size_t o_jobs = L.add(sizeof(BlockJob) * nb);
-    const size_t o_pjobs = L.add(sizeof(PcrdJob) * nb);
+    // The readback group (selection, results, totals, lambdas) is laid out
+    // in the order of its host buffers, so one copy returns it.
+    const size_t o_sel = L.add(sizeof(Selection) * nb);
     const size_t o_res = L.add(sizeof(BlockResult) * nb);
+    const size_t o_totals = L.add(4 * imgs.size());
+    const size_t o_lambda = L.add(8 * ngroups);
     const size_t o_snaps = L.add(sizeof(uint32_t) * kMaxPasses * nb);
     const size_t o_ns = L.add(sizeof(uint16_t) * kMaxPasses * nb);
     const size_t o_slots = L.add(slot_total);
     const size_t o_hull = L.add(sizeof(HullPt) * (kMaxPasses + 1) * nb);
     const size_t o_hulln = L.add(nb);
-    const size_t o_gmax = L.add(8 * ngroups);
-    const size_t o_rawf = L.add(16 * ngroups);
-    const size_t o_lambda = L.add(8 * ngroups);
-    const size_t o_sel = L.add(sizeof(Selection) * nb);
     const size_t o_offs = L.add(4 * nb);
-    const size_t o_totals = L.add(4 * imgs.size());
-    const size_t o_imgfirst = L.add(4 * img_first.size());
-    const size_t o_blkimg = L.add(4 * nb);
-    const size_t o_packbase = L.add(8 * imgs.size());
     const size_t o_packed = L.add(pack_total);
     const size_t o_sched = L.add(4 * t1_sched_words(nb));
     // Lane-per-block Tier-1 (large batches): symbol streams, bounded per
@@ -375,10 +392,22 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
             }
         }
     }
+    // The uploaded tables follow each other in the order they are staged in
+    // host memory (the same sizes and alignment on both sides), so their
+    // copies merge into one.
+    const size_t o_jobs = L.add(sizeof(BlockJob) * nb);
+    const size_t o_pjobs = L.add(sizeof(PcrdJob) * nb);
+    const size_t o_gmax = L.add(8 * ngroups);
+    const size_t o_rawf = L.add(16 * ngroups);
+    const size_t o_imgfirst = L.add(4 * img_first.size());
+    const size_t o_blkimg = L.add(4 * nb);
+    const size_t o_packbase = L.add(8 * imgs.size());
     size_t o_pseg_k[2], o_fseg_k[2][kMaxLevels];
     for (int kind = 0; kind < 2; kind++) {
-        o_pseg_k[kind] = L.add(sizeof(PixSeg) * std::max<size_t>(pseg_k[kind].size(), 1));
-        for (int k = 0; k < maxL; k++) o_fseg_k[kind][k] = L.add(sizeof(DwtSeg) * std::max<size_t>(fseg_k[kind][k].size(), 1));
+        // (an empty table takes no space, as in the host staging; its
+        // launch is skipped)
+        o_pseg_k[kind] = L.add(sizeof(PixSeg) * pseg_k[kind].size());
+        for (int k = 0; k < maxL; k++) o_fseg_k[kind][k] = L.add(sizeof(DwtSeg) * fseg_k[kind][k].size());
     }
     // encode_with_psnr (backend.rs:305-433): the lossy images' blocks are
     // dequantised as truncated in place, inverted into their x planes
@@ -480,10 +509,6 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         if (bytes) J2L_HIP(hipMemcpyAsync(D + off, src, bytes, hipMemcpyHostToDevice, st));
     };
     Layout H;
-    struct Staged {
-        size_t dst, hoff, n;
-        const void* src;
-    };
     std::vector<Staged> staged;
     auto stage = [&](size_t off, const void* src, size_t bytes) {
         if (bytes) staged.push_back({off, H.add(bytes), bytes, src});
@@ -506,10 +531,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     const size_t h_lam = H.add(8 * ngroups);
     pin_.reserve(H.used);  // once per call: DMAs from it stay in flight
     char* P = pin_.base;
-    for (auto& u : staged) {
-        std::memcpy(P + u.hoff, u.src, u.n);
-        J2L_HIP(hipMemcpyAsync(D + u.dst, P + u.hoff, u.n, hipMemcpyHostToDevice, st));
-    }
+    copy_staged(staged, D, P, st);
     for (auto& im : imgs)
         if (im.in_bytes) up(im.in_off, im.src, im.in_bytes);
 
@@ -603,10 +625,14 @@ t1_again:
     J2L_HIP(hipEventRecord(ev.e[3], st));
 
     // ---- readback ----------------------------------------------------------------------
-    J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, sizeof(Selection) * nb, hipMemcpyDeviceToHost, st));
-    J2L_HIP(hipMemcpyAsync(P + h_res, D + o_res, sizeof(BlockResult) * nb, hipMemcpyDeviceToHost, st));
-    J2L_HIP(hipMemcpyAsync(P + h_tot, D + o_totals, 4 * imgs.size(), hipMemcpyDeviceToHost, st));
-    J2L_HIP(hipMemcpyAsync(P + h_lam, D + o_lambda, 8 * ngroups, hipMemcpyDeviceToHost, st));
+    if (o_res - o_sel == h_res - h_sel && o_totals - o_sel == h_tot - h_sel && o_lambda - o_sel == h_lam - h_sel) {
+        J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, o_lambda + 8 * ngroups - o_sel, hipMemcpyDeviceToHost, st));
+    } else {
+        J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, sizeof(Selection) * nb, hipMemcpyDeviceToHost, st));
+        J2L_HIP(hipMemcpyAsync(P + h_res, D + o_res, sizeof(BlockResult) * nb, hipMemcpyDeviceToHost, st));
+        J2L_HIP(hipMemcpyAsync(P + h_tot, D + o_totals, 4 * imgs.size(), hipMemcpyDeviceToHost, st));
+        J2L_HIP(hipMemcpyAsync(P + h_lam, D + o_lambda, 8 * ngroups, hipMemcpyDeviceToHost, st));
+    }
     J2L_HIP(hipStreamSynchronize(st));
     t_dwt_ms = ev.ms(0, 1);
     t_t1_ms = ev.ms(1, 2);
@@ -992,7 +1018,6 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         }
     }
     const int nj = (int)jobs.size();
-    const size_t o_jobs = L.add(sizeof(DecodeJob) * nj);
     // inverse segment tables per step k (coarsest first)
     std::vector<std::vector<DwtSeg>> isegs(2 * std::max(maxL, 1));
     std::vector<PixSeg> fsegs[2];
@@ -1032,8 +1057,12 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     std::vector<size_t> o_i32(imgs.size(), 0);
     for (size_t k = 0; k < imgs.size(); k++)
         if (items[imgs[k].idx].target == DST_HOST_I32) o_i32[k] = L.add(imgs[k].n * 4 * imgs[k].ci.ncomp);
+    // jobs and segment tables follow each other in their staging order, so
+    // their uploads merge into one copy (an empty table takes no space; its
+    // launch is skipped)
+    const size_t o_jobs = L.add(sizeof(DecodeJob) * nj);
     std::vector<size_t> o_iseg(isegs.size());
-    for (size_t q = 0; q < isegs.size(); q++) o_iseg[q] = L.add(sizeof(DwtSeg) * std::max<size_t>(isegs[q].size(), 1));
+    for (size_t q = 0; q < isegs.size(); q++) o_iseg[q] = L.add(sizeof(DwtSeg) * isegs[q].size());
     const size_t o_fseg0 = L.add(sizeof(PixSeg) * std::max<size_t>(fsegs[0].size(), 1));
     const size_t o_fseg1 = L.add(sizeof(PixSeg) * std::max<size_t>(fsegs[1].size(), 1));
 
@@ -1075,10 +1104,6 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     // payloads and tables go through the page-locked arena (one asynchronous
     // DMA each; a pageable source makes the runtime stage it synchronously)
     Layout H;
-    struct Staged {
-        size_t dst, hoff, n;
-        const void* src;
-    };
     std::vector<Staged> staged;
     auto stage = [&](size_t off, const void* src, size_t bytes) {
         if (bytes) staged.push_back({off, H.add(bytes), bytes, src});
@@ -1102,10 +1127,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     stage(o_fseg0, fsegs[0].data(), sizeof(PixSeg) * fsegs[0].size());
     stage(o_fseg1, fsegs[1].data(), sizeof(PixSeg) * fsegs[1].size());
     pin_.reserve(H.used);
-    for (auto& u : staged) {
-        std::memcpy(pin_.base + u.hoff, u.src, u.n);
-        J2L_HIP(hipMemcpyAsync(D + u.dst, pin_.base + u.hoff, u.n, hipMemcpyHostToDevice, st));
-    }
+    copy_staged(staged, D, pin_.base, st);
 
     Events ev;
     J2L_HIP(hipEventRecord(ev.e[0], st));
