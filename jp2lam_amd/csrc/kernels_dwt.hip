@@ -144,7 +144,7 @@ __device__ __forceinline__ int find_seg(const DwtSeg* segs, int nseg, uint32_t b
 // Subbands: LL -> out, HL/TLH/HH -> plane at their Mallat offsets.  Tile
 // origins are multiples of 64 so tile-local parity equals global parity.
 template <typename T, bool IRREV>
-__global__ __launch_bounds__(256) void fwd_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
+__global__ __launch_bounds__(1024) void fwd_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
     __shared__ T s[TLH][TLP];
     const DwtSeg S = segs[find_seg(segs, nseg, blockIdx.x)];
     const uint32_t t = blockIdx.x - S.tile_start;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void fwd_level_kernel(const DwtSeg* __restrict
 // One inverse level: LL from `in`, HL/TLH/HH from `plane`, interleaved
 // rw x rh result into `out` (irrev97.rs:636-820; rev53.rs:95-136, :272-707).
 template <typename T, bool IRREV>
-__global__ __launch_bounds__(256) void inv_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
+__global__ __launch_bounds__(1024) void inv_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
     __shared__ T s[TLH][TLP];
     const DwtSeg S = segs[find_seg(segs, nseg, blockIdx.x)];
     const uint32_t t = blockIdx.x - S.tile_start;
@@ -396,15 +396,21 @@ __global__ __launch_bounds__(256) void finalize_kernel(const PixSeg* __restrict_
 }
 
 // ---- host launchers ---------------------------------------------------------------
+// A level with few tiles (the coarse ones) lasts as long as one tile's
+// chain of lifting steps and barriers, so its tiles get 1024 threads; the
+// wide levels keep 256 (each thread's share of a step is then 4x smaller).
+inline int level_threads(uint32_t ntiles) { return ntiles < 256 ? 1024 : 256; }
 void launch_fwd_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles, hipStream_t st) {
     if (!ntiles) return;
-    if (irrev) fwd_level_kernel<float, true><<<ntiles, 256, 0, st>>>(segs, nseg);
-    else fwd_level_kernel<int32_t, false><<<ntiles, 256, 0, st>>>(segs, nseg);
+    const int nt = level_threads(ntiles);
+    if (irrev) fwd_level_kernel<float, true><<<ntiles, nt, 0, st>>>(segs, nseg);
+    else fwd_level_kernel<int32_t, false><<<ntiles, nt, 0, st>>>(segs, nseg);
 }
 void launch_inv_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles, hipStream_t st) {
     if (!ntiles) return;
-    if (irrev) inv_level_kernel<float, true><<<ntiles, 256, 0, st>>>(segs, nseg);
-    else inv_level_kernel<int32_t, false><<<ntiles, 256, 0, st>>>(segs, nseg);
+    const int nt = level_threads(ntiles);
+    if (irrev) inv_level_kernel<float, true><<<ntiles, nt, 0, st>>>(segs, nseg);
+    else inv_level_kernel<int32_t, false><<<ntiles, nt, 0, st>>>(segs, nseg);
 }
 void launch_prepare(const PixSeg* segs, int nseg, uint32_t nblocks, hipStream_t st) {
     if (nblocks) prepare_kernel<<<nblocks, 256, 0, st>>>(segs, nseg);
