@@ -26,27 +26,49 @@ __device__ __forceinline__ double class_weight(int cls, bool ll) {
 }  // namespace
 
 // Per-block pass lengths (t1.rs:586-600), distortion deltas and convex hull.
-// One thread per block.  `hull` has kMaxPasses + 1 points per block.
-__global__ void pcrd_hull_kernel(int n, const BlockResult* __restrict__ res, const uint32_t* __restrict__ snaps,
-                                 const uint16_t* __restrict__ newsigs, const PcrdJob* __restrict__ jobs,
-                                 HullPt* __restrict__ hull, uint8_t* __restrict__ hull_n,
-                                 unsigned long long* __restrict__ gmax) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
+// G (1 or kHullG) blocks per wave: the lanes bring the blocks' pass records
+// into LDS together, then lane 8g runs block g's hull with its stack in LDS
+// (a per-thread stack would sit in scratch, and every pass would wait a
+// memory latency).  One block per wave is the fastest for a page; large
+// batches take 8 per wave to keep the wave count down.  `hull` has
+// kMaxPasses + 1 points per block.
+constexpr int kHullG = 8;
+__global__ __launch_bounds__(64) void pcrd_hull_kernel(int n, int G, const BlockResult* __restrict__ res,
+                                                       const uint32_t* __restrict__ snaps,
+                                                       const uint16_t* __restrict__ newsigs,
+                                                       const PcrdJob* __restrict__ jobs, HullPt* __restrict__ hull,
+                                                       uint8_t* __restrict__ hull_n,
+                                                       unsigned long long* __restrict__ gmax) {
+    __shared__ uint32_t s_sp[kHullG][kMaxPasses], s_hb[kHullG][kMaxPasses + 1];
+    __shared__ uint16_t s_ns[kHullG][kMaxPasses];
+    __shared__ double s_hd[kHullG][kMaxPasses + 1];
+    __shared__ int s_hp[kHullG][kMaxPasses + 1];
+    const int b0 = blockIdx.x * G;
+    for (int g = 0; g < G && b0 + g < n; g++) {
+        const int npg = min((int)res[b0 + g].npasses, kMaxPasses);
+        for (int i = threadIdx.x; i < npg; i += blockDim.x) {
+            s_sp[g][i] = snaps[(size_t)(b0 + g) * kMaxPasses + i];
+            s_ns[g][i] = newsigs[(size_t)(b0 + g) * kMaxPasses + i];
+        }
+    }
+    __syncthreads();
+    const int g = (int)threadIdx.x >> 3;
+    const int b = b0 + g;
+    if ((threadIdx.x & 7u) != 0 || g >= G || b >= n) return;
+    const uint32_t* sp = s_sp[g];
+    const uint16_t* ns = s_ns[g];
+    uint32_t* hb = s_hb[g];
+    double* hd = s_hd[g];
+    int* hp = s_hp[g];
     const BlockResult R = res[b];
-    const PcrdJob J = jobs[b];
     const int np = R.npasses;
+    const PcrdJob J = jobs[b];
     HullPt* H = hull + (size_t)b * (kMaxPasses + 1);
-    uint32_t hb[kMaxPasses + 1];
-    double hd[kMaxPasses + 1];
-    int hp[kMaxPasses + 1];
     int m = 0;
     hb[0] = 0;
     hd[0] = 0.0;
     hp[0] = 0;
     m = 1;
-    const uint32_t* sp = snaps + (size_t)b * kMaxPasses;
-    const uint16_t* ns = newsigs + (size_t)b * kMaxPasses;
     uint32_t prev = 0, sum = 0, cum = 0;
     uint32_t sig_total = 0, sig_before_sp = 0;
     double cumd = 0.0;
@@ -215,7 +237,8 @@ void launch_pcrd(int n, bool lossless, int ngroups, const BlockResult* res, cons
                  unsigned long long* gmax, const double* raw_floor, double* lambda, Selection* sel, hipStream_t st) {
     if (n <= 0) return;
     if (!lossless) {
-        pcrd_hull_kernel<<<(n + 127) / 128, 128, 0, st>>>(n, res, snaps, newsigs, jobs, hull, hull_n, gmax);
+        const int G = n <= 4096 ? 1 : kHullG;
+        pcrd_hull_kernel<<<(n + G - 1) / G, 64, 0, st>>>(n, G, res, snaps, newsigs, jobs, hull, hull_n, gmax);
         pcrd_lambda_kernel<<<(ngroups + 63) / 64, 64, 0, st>>>(ngroups, gmax, raw_floor, lambda);
     }
     pcrd_select_kernel<<<(n + 127) / 128, 128, 0, st>>>(n, lossless ? 1 : 0, res, snaps, jobs, hull, hull_n, lambda, sel);
