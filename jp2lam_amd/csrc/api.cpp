@@ -501,7 +501,7 @@ jp2lam_status jp2g_forward_dwt(const uint8_t* pixels, uint32_t width, uint32_t h
             ps.comp = (int)c;
             ps.irrev = irrev;
             J2L_HIP(hipMemcpyAsync(dps, &ps, sizeof(ps), hipMemcpyHostToDevice, st));
-            launch_prepare(dps, 1, (uint32_t)((n + 255) / 256), st);
+            launch_prepare(dps, 1, pix_blocks(n), st);
             if (p.levels == 0) J2L_HIP(hipMemcpyAsync(P, X, n * 4, hipMemcpyDeviceToDevice, st));
             for (int k = 0; k < p.levels; k++) {
                 const int Lv = p.levels;
@@ -586,7 +586,7 @@ jp2lam_status jp2g_inverse_dwt(const void* planes, uint32_t width, uint32_t heig
         f.channels = (int)channels;
         f.irrev = irreversible != 0;
         J2L_HIP(hipMemcpyAsync(dps, &f, sizeof(f), hipMemcpyHostToDevice, st));
-        launch_finalize(dps, 1, (uint32_t)((n + 255) / 256), st);
+        launch_finalize(dps, 1, pix_blocks(n), st);
         J2L_HIP(hipMemcpyAsync(out, d8, n * channels, hipMemcpyDeviceToHost, st));
         J2L_HIP(hipStreamSynchronize(st));
         (void)hipFree(P);

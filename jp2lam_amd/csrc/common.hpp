@@ -37,6 +37,10 @@ struct Error : std::runtime_error {
     } while (0)
 
 constexpr int kMaxLevels = 6;      // encoder caps: 5 gray / 6 sRGB (plan/derive.rs:109-115)
+// Pixels per workgroup of the colour kernels (prepare, finalize): 256
+// threads, four coalesced passes each.
+constexpr uint32_t kPixPerBlock = 1024;
+inline uint32_t pix_blocks(size_t n) { return (uint32_t)((n + kPixPerBlock - 1) / kPixPerBlock); }
 constexpr int kMaxPasses = 96;     // 1 + 3 * (31 - 1) rounded up
 constexpr int kMaxDecodeLevels = 32;
 

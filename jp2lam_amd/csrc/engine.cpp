@@ -363,7 +363,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
             s.planar_i32 = im.kind == SRC_HOST_I32;
             s.irrev = kind;
             s.blk_start = pblk_k[kind];
-            pblk_k[kind] += (uint32_t)((im.n + 255) / 256);
+            pblk_k[kind] += pix_blocks(im.n);
             pseg_k[kind].push_back(s);
         }
     }
@@ -1096,7 +1096,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
             else if (it.target == DST_HOST_U8) f.dst = A(im.u8_off);
             if (it.target == DST_HOST_I32) f.dst_i32 = A(o_i32[k]);
             f.blk_start = fblocks[kind];
-            fblocks[kind] += (uint32_t)((im.n + 255) / 256);
+            fblocks[kind] += pix_blocks(im.n);
         }
     }
 

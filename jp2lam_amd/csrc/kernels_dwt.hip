@@ -307,7 +307,9 @@ __device__ __forceinline__ int find_pix(const PixSeg* segs, int nseg, uint32_t b
 // forward RCT (scalar.rs:280-302) into a component plane.
 __global__ __launch_bounds__(256) void prepare_kernel(const PixSeg* __restrict__ segs, int nseg) {
     const PixSeg S = segs[find_pix(segs, nseg, blockIdx.x)];
-    const size_t i = (size_t)(blockIdx.x - S.blk_start) * 256 + threadIdx.x;
+    const size_t i0 = (size_t)(blockIdx.x - S.blk_start) * kPixPerBlock + threadIdx.x;
+    for (uint32_t c256 = 0; c256 < kPixPerBlock / 256u; c256++) {
+    const size_t i = i0 + (size_t)c256 * 256u;  // each pass a coalesced 256
     if (i >= S.n) return;
     int r, g, b;
     if (S.planar_i32) {
@@ -351,13 +353,16 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PixSeg* __restrict__
         }
         ((int32_t*)S.dst)[i] = v;
     }
+    }
 }
 
 // Inverse ICT/RCT and round/clamp to 0..255 (scalar.rs:197-236; f32::round
 // is half away from zero).
 __global__ __launch_bounds__(256) void finalize_kernel(const PixSeg* __restrict__ segs, int nseg) {
     const PixSeg S = segs[find_pix(segs, nseg, blockIdx.x)];
-    const size_t i = (size_t)(blockIdx.x - S.blk_start) * 256 + threadIdx.x;
+    const size_t i0 = (size_t)(blockIdx.x - S.blk_start) * kPixPerBlock + threadIdx.x;
+    for (uint32_t c256 = 0; c256 < kPixPerBlock / 256u; c256++) {
+    const size_t i = i0 + (size_t)c256 * 256u;  // each pass a coalesced 256
     if (i >= S.n) return;
     const int ch = S.channels;
     int v[3];
@@ -392,6 +397,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(const PixSeg* __restrict_
     for (int c = 0; c < ch; c++) {
         if (o8) o8[i * ch + c] = (uint8_t)v[c];
         if (o32) o32[(size_t)c * S.n + i] = v[c];
+    }
     }
 }
 
