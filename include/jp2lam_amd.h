@@ -33,11 +33,12 @@ typedef enum {
     JP2LAM_IO = 5
 } jp2lam_status;
 
-/* ColorSpace (src/model.rs:1-12). */
+/* ColorSpace (src/model.rs:1-12), numbered in the enum's declaration order
+ * so that `ColorSpace as u32` on the Rust side is the value passed here. */
 typedef enum {
     JP2LAM_GRAY = 0,
-    JP2LAM_SRGB = 1,
-    JP2LAM_RGB = 2,
+    JP2LAM_RGB = 1,
+    JP2LAM_SRGB = 2,
     JP2LAM_YUV = 3,
     JP2LAM_YCBCR = 4
 } jp2lam_colorspace;
@@ -144,11 +145,32 @@ jp2lam_status jp2lam_batch_encoder_encode_one(jp2lam_batch_encoder* enc, const j
 size_t jp2lam_batch_encoder_encoded_count(const jp2lam_batch_encoder* enc);
 int jp2lam_batch_encoder_has_profile(const jp2lam_batch_encoder* enc);
 void jp2lam_batch_encoder_free(jp2lam_batch_encoder* enc);
+/* BatchComponentProfile (src/batch.rs:15-23). */
+typedef struct {
+    uint32_t width, height, precision, is_signed, dx, dy;
+} jp2lam_component_profile;
+/* BatchProfile (src/batch.rs:5-13) without its component list, which the
+ * getters below write to a caller array. */
+typedef struct {
+    uint32_t width, height, colorspace, num_components;
+    uint8_t quality, format, pad[2];
+} jp2lam_batch_profile;
+/* BatchEncoder::profile (src/batch.rs:47-49): 1 and *out filled (plus the
+ * first min(cap, num_components) component profiles into comps, which may
+ * be NULL when cap is 0) once the first item has fixed the profile; 0 while
+ * it is None; -1 for a null handle or out. */
+int jp2lam_batch_encoder_profile(const jp2lam_batch_encoder* enc, jp2lam_batch_profile* out,
+                                 jp2lam_component_profile* comps, size_t cap);
 jp2lam_batch_decoder* jp2lam_batch_decoder_new(void);
 jp2lam_status jp2lam_batch_decoder_decode_one(jp2lam_batch_decoder* dec, const uint8_t* bytes, size_t len,
                                               jp2lam_image* out);
 size_t jp2lam_batch_decoder_decoded_count(const jp2lam_batch_decoder* dec);
 void jp2lam_batch_decoder_free(jp2lam_batch_decoder* dec);
+/* BatchDecoder::profile (src/batch.rs:91-93); same contract as
+ * jp2lam_batch_encoder_profile.  A decoder's profile carries quality 0 and
+ * format JP2 (BatchProfile::for_decode, src/batch.rs:152-154). */
+int jp2lam_batch_decoder_profile(const jp2lam_batch_decoder* dec, jp2lam_batch_profile* out,
+                                 jp2lam_component_profile* comps, size_t cap);
 
 /* ---- mixed batches (no counterpart in the reference) ------------------------
  * Pages of different sizes and qualities in one device batch; each result

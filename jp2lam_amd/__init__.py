@@ -27,7 +27,7 @@ __all__ = [
     "ColorSpace", "OutputFormat", "Preset", "EncodeOptions", "Component", "Image", "DecodeMetadata",
     "Jp2LamError", "InvalidInput", "EncodeFailed", "DecodeFailed", "UnsupportedFeature", "IoError",
     "encode", "encode_to_writer", "encode_with_psnr", "EncodeMetrics", "decode_jp2", "decode_from_reader", "inspect_jp2", "BatchEncoder",
-    "BatchDecoder", "encode_batch", "decode_batch", "encode_u8", "decode_u8", "encode_many", "decode_many",
+    "BatchDecoder", "BatchProfile", "BatchComponentProfile", "encode_batch", "decode_batch", "encode_u8", "decode_u8", "encode_many", "decode_many",
     "Server", "Served", "ContentClass", "ContentStats", "analyze_content", "auto_quality",
     "auto_class_and_quality", "analyze_content_u8", "PHOTO_QUALITY", "GRAPHICS_QUALITY",
 ]
@@ -35,9 +35,10 @@ __all__ = [
 
 # ---- model (src/model.rs) ---------------------------------------------------------
 class ColorSpace(enum.IntEnum):
+    """ColorSpace in the declaration order of src/model.rs:1-12."""
     GRAY = 0
-    SRGB = 1
-    RGB = 2
+    RGB = 1
+    SRGB = 2
     YUV = 3
     YCBCR = 4
 
@@ -365,6 +366,44 @@ def inspect_jp2(data) -> DecodeMetadata:
                           m.num_quant_steps, m.tile_part_count, m.first_tile_payload_len)
 
 
+@dataclass(frozen=True)
+class BatchComponentProfile:
+    """BatchComponentProfile (src/batch.rs:15-23)."""
+    width: int
+    height: int
+    precision: int
+    signed: bool
+    dx: int
+    dy: int
+
+
+@dataclass(frozen=True)
+class BatchProfile:
+    """BatchProfile (src/batch.rs:5-13)."""
+    width: int
+    height: int
+    colorspace: ColorSpace
+    components: tuple
+    quality: int
+    format: OutputFormat
+
+
+def _read_profile(getter, handle) -> Optional[BatchProfile]:
+    p = _lib.BatchProfileC()
+    comps = (_lib.ComponentProfileC * 8)()
+    rc = getter(handle, C.byref(p), comps, len(comps))
+    if rc < 0:
+        raise InvalidInput("null batch handle")
+    if rc == 0:
+        return None
+    if p.num_components > len(comps):
+        comps = (_lib.ComponentProfileC * p.num_components)()
+        getter(handle, C.byref(p), comps, len(comps))
+    cs = tuple(BatchComponentProfile(k.width, k.height, k.precision, bool(k.is_signed), k.dx, k.dy)
+               for k in comps[:p.num_components])
+    return BatchProfile(p.width, p.height, ColorSpace(p.colorspace), cs, p.quality, OutputFormat(p.format))
+
+
 class BatchEncoder:
     """BatchEncoder (src/batch.rs:32-84): the first image fixes the profile."""
 
@@ -394,6 +433,10 @@ class BatchEncoder:
     def has_profile(self) -> bool:
         return bool(self._lib.jp2lam_batch_encoder_has_profile(self._h))
 
+    def profile(self) -> Optional["BatchProfile"]:
+        """BatchEncoder::profile (src/batch.rs:47-49)."""
+        return _read_profile(self._lib.jp2lam_batch_encoder_profile, self._h)
+
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
         if h:
@@ -421,6 +464,10 @@ class BatchDecoder:
     @property
     def decoded_count(self) -> int:
         return int(self._lib.jp2lam_batch_decoder_decoded_count(self._h))
+
+    def profile(self) -> Optional["BatchProfile"]:
+        """BatchDecoder::profile (src/batch.rs:91-93)."""
+        return _read_profile(self._lib.jp2lam_batch_decoder_profile, self._h)
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None

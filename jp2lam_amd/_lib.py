@@ -53,6 +53,16 @@ class DecodeItemC(C.Structure):
                 ("on_device", C.c_uint8), ("pad", C.c_uint8 * 3), ("status", C.c_int32)]
 
 
+class ComponentProfileC(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("width", "height", "precision", "is_signed", "dx", "dy")]
+
+
+class BatchProfileC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("colorspace", C.c_uint32),
+                ("num_components", C.c_uint32), ("quality", C.c_uint8), ("format", C.c_uint8),
+                ("pad", C.c_uint8 * 2)]
+
+
 class MetricsC(C.Structure):
     _fields_ = [("psnr_db", C.c_double), ("ssim", C.c_double)]
 
@@ -90,10 +100,14 @@ SIGNATURES = [
     ("jp2lam_batch_encoder_encoded_count", C.c_size_t, [C.c_void_p]),
     ("jp2lam_batch_encoder_has_profile", C.c_int, [C.c_void_p]),
     ("jp2lam_batch_encoder_free", None, [C.c_void_p]),
+    ("jp2lam_batch_encoder_profile", C.c_int, [C.c_void_p, C.POINTER(BatchProfileC), C.POINTER(ComponentProfileC),
+                                              C.c_size_t]),
     ("jp2lam_batch_decoder_new", C.c_void_p, []),
     ("jp2lam_batch_decoder_decode_one", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(ImageC)]),
     ("jp2lam_batch_decoder_decoded_count", C.c_size_t, [C.c_void_p]),
     ("jp2lam_batch_decoder_free", None, [C.c_void_p]),
+    ("jp2lam_batch_decoder_profile", C.c_int, [C.c_void_p, C.POINTER(BatchProfileC), C.POINTER(ComponentProfileC),
+                                              C.c_size_t]),
     ("jp2lam_encode_many", C.c_int, [C.POINTER(EncodeItemC), C.c_size_t]),
     ("jp2lam_decode_many", C.c_int, [C.POINTER(DecodeItemC), C.c_size_t]),
     ("jp2g_forward_dwt", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),

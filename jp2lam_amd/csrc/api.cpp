@@ -162,6 +162,23 @@ Profile profile_of(const jp2lam_image_view* im, int quality, int format) {
     return p;
 }
 
+int export_profile(const std::optional<Profile>& p, jp2lam_batch_profile* out, jp2lam_component_profile* comps,
+                   size_t cap) {
+    std::memset(out, 0, sizeof(*out));
+    if (!p) return 0;
+    out->width = p->width;
+    out->height = p->height;
+    out->colorspace = p->colorspace;
+    out->num_components = (uint32_t)p->comps.size();
+    out->quality = (uint8_t)p->quality;
+    out->format = (uint8_t)p->format;
+    for (size_t c = 0; c < p->comps.size() && c < cap && comps; c++) {
+        const auto& k = p->comps[c];
+        comps[c] = {k[0], k[1], k[2], k[3], k[4], k[5]};
+    }
+    return 1;
+}
+
 }  // namespace
 
 struct jp2lam_batch_encoder {
@@ -360,6 +377,11 @@ jp2lam_status jp2lam_batch_encoder_encode_one(jp2lam_batch_encoder* enc, const j
 size_t jp2lam_batch_encoder_encoded_count(const jp2lam_batch_encoder* enc) { return enc ? enc->count : 0; }
 int jp2lam_batch_encoder_has_profile(const jp2lam_batch_encoder* enc) { return enc && enc->profile ? 1 : 0; }
 void jp2lam_batch_encoder_free(jp2lam_batch_encoder* enc) { delete enc; }
+int jp2lam_batch_encoder_profile(const jp2lam_batch_encoder* enc, jp2lam_batch_profile* out,
+                                 jp2lam_component_profile* comps, size_t cap) {
+    if (!enc || !out) return -1;
+    return export_profile(enc->profile, out, comps, cap);
+}
 
 jp2lam_batch_decoder* jp2lam_batch_decoder_new(void) { return new (std::nothrow) jp2lam_batch_decoder(); }
 
@@ -389,6 +411,11 @@ jp2lam_status jp2lam_batch_decoder_decode_one(jp2lam_batch_decoder* dec, const u
 
 size_t jp2lam_batch_decoder_decoded_count(const jp2lam_batch_decoder* dec) { return dec ? dec->count : 0; }
 void jp2lam_batch_decoder_free(jp2lam_batch_decoder* dec) { delete dec; }
+int jp2lam_batch_decoder_profile(const jp2lam_batch_decoder* dec, jp2lam_batch_profile* out,
+                                 jp2lam_component_profile* comps, size_t cap) {
+    if (!dec || !out) return -1;
+    return export_profile(dec->profile, out, comps, cap);
+}
 
 jp2lam_status jp2lam_encode_many(jp2lam_encode_item* items, size_t n) {
     return guarded([&] {

@@ -927,42 +927,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         try {
             Img im;
             im.idx = (int)i;
-            const uint8_t* cs;
-            size_t cslen;
-            parse_jp2(items[i].bytes, items[i].len, im.jp, cs, cslen);
-            parse_codestream(cs, cslen, im.ci);
-            // decode/mod.rs:81-129
-            if (im.jp.width != im.ci.width || im.jp.height != im.ci.height)
-                throw Error(DECODE_FAILED, "JP2 ihdr dimensions do not match SIZ dimensions");
-            if (im.jp.bpc != 8)
-                throw Error(UNSUPPORTED_FEATURE, "unsupported JP2 bit depth: decoder currently supports 8-bit components, found " +
-                                                      std::to_string(im.jp.bpc) + " bits");
-            if (im.jp.ncomp != im.ci.ncomp) throw Error(DECODE_FAILED, "JP2 ihdr component count does not match SIZ component count");
-            if (im.jp.colorspace == GRAY && im.jp.ncomp != 1)
-                throw Error(UNSUPPORTED_FEATURE, "unsupported JP2 component count: decoder currently supports one grayscale component");
-            if (im.jp.colorspace == SRGB && im.jp.ncomp != 3)
-                throw Error(UNSUPPORTED_FEATURE, "unsupported JP2 component count: decoder currently supports three sRGB components");
-            if (im.ci.width > 65535 || im.ci.height > 65535)
-                throw Error(UNSUPPORTED_FEATURE, "image dimensions above 65535 are not supported by the device path");
-            if (im.ci.cbw > 64 || im.ci.cbh > 64)
-                throw Error(UNSUPPORTED_FEATURE, "code-blocks larger than 64x64 are not supported by the device path");
-            parse_packets(im.ci, im.blocks);
-            // decode/t1.rs:96-127 per-block checks, in contribution order
-            const size_t nsteps = im.ci.exps.size();
-            for (auto& b : im.blocks) {
-                if (!b.included || b.passes == 0) continue;
-                int mb = std::max(im.ci.guard - 1, 0) + im.ci.exps[b.band_index % nsteps];
-                mb &= 255;
-                if (b.zbp > 255) throw Error(DECODE_FAILED, "zero-bitplane count exceeds supported range");
-                if (b.zbp > mb) throw Error(DECODE_FAILED, "zero-bitplane count exceeds subband bit-plane count");
-                int mbp = mb - b.zbp;
-                if (mbp == 0) throw Error(DECODE_FAILED, "non-empty code-block has no magnitude bit-planes");
-                int maxp = 1 + 3 * (mbp - 1);
-                if (b.passes > maxp)
-                    throw Error(DECODE_FAILED, "code-block pass count " + std::to_string(b.passes) + " exceeds maximum " +
-                                                   std::to_string(maxp) + " for " + std::to_string(mbp) + " magnitude bit-planes");
-                if (mbp > 31) throw Error(UNSUPPORTED_FEATURE, "more than 31 magnitude bit-planes");
-            }
+            parse_for_decode(items[i].bytes, items[i].len, im.jp, im.ci, im.blocks);
             im.n = (size_t)im.ci.width * im.ci.height;
             resolution_ladder(im.ci.width, im.ci.height, im.ci.levels, im.rw, im.rh);
             if (items[i].target == DST_DEV_U8 && items[i].dev_cap < im.n * im.ci.ncomp)

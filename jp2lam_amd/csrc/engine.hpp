@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "common.hpp"
 #include "host.hpp"
 
 namespace j2l {

@@ -745,7 +745,10 @@ void parse_codestream(const uint8_t* b, size_t n, CodestreamInfo& ci) {
             uint16_t m = rd16(b + pos);
             if (m != 0xff64) unsupported("unsupported marker in tile-part header");
             if (n - pos < 4) fail("truncated marker segment");
-            pos += 2 + rd16(b + pos + 2);
+            size_t L = rd16(b + pos + 2);
+            if (L < 2) fail("invalid marker length");
+            if (L > n - pos - 2) fail("marker extended past end of codestream");
+            pos += 2 + L;
         }
         if (n - pos < 2) fail("tile-part lacks SOD");
         pos += 2;
@@ -907,6 +910,51 @@ void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
                 }
             }
     if (pos != plen) fail("tile payload has " + std::to_string(plen - pos) + " trailing bytes after packet parse");
+}
+
+
+// decode/t1.rs:96-127: per-block checks, in contribution order, before any
+// block reaches the device decoder.
+void check_decode_blocks(const CodestreamInfo& ci, const std::vector<DecBlock>& blocks) {
+    const size_t nsteps = ci.exps.size();
+    if (nsteps == 0) fail("QCD step count does not match decomposition levels");
+    for (const auto& b : blocks) {
+        if (!b.included || b.passes == 0) continue;
+        int mb = std::max(ci.guard - 1, 0) + ci.exps[(size_t)b.band_index % nsteps];
+        mb &= 255;
+        if (b.zbp > 255) fail("zero-bitplane count exceeds supported range");
+        if (b.zbp > mb) fail("zero-bitplane count exceeds subband bit-plane count");
+        int mbp = mb - b.zbp;
+        if (mbp == 0) fail("non-empty code-block has no magnitude bit-planes");
+        int maxp = 1 + 3 * (mbp - 1);
+        if (b.passes > maxp)
+            fail("code-block pass count " + std::to_string(b.passes) + " exceeds maximum " + std::to_string(maxp) +
+                 " for " + std::to_string(mbp) + " magnitude bit-planes");
+        if (mbp > 31) unsupported("more than 31 magnitude bit-planes");
+    }
+}
+
+// decode/mod.rs:56-129 up to the Tier-1 decode: every host-side step of
+// decode_jp2, with the device path's own limits.
+void parse_for_decode(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
+    const uint8_t* cs;
+    size_t cslen;
+    parse_jp2(bytes, len, jp, cs, cslen);
+    parse_codestream(cs, cslen, ci);
+    if (jp.width != ci.width || jp.height != ci.height) fail("JP2 ihdr dimensions do not match SIZ dimensions");
+    if (jp.bpc != 8)
+        unsupported("unsupported JP2 bit depth: decoder currently supports 8-bit components, found " +
+                    std::to_string(jp.bpc) + " bits");
+    if (jp.ncomp != ci.ncomp) fail("JP2 ihdr component count does not match SIZ component count");
+    if (jp.colorspace == GRAY && jp.ncomp != 1)
+        unsupported("unsupported JP2 component count: decoder currently supports one grayscale component");
+    if (jp.colorspace == SRGB && jp.ncomp != 3)
+        unsupported("unsupported JP2 component count: decoder currently supports three sRGB components");
+    if (ci.width > 65535 || ci.height > 65535)
+        unsupported("image dimensions above 65535 are not supported by the device path");
+    if (ci.cbw > 64 || ci.cbh > 64) unsupported("code-blocks larger than 64x64 are not supported by the device path");
+    parse_packets(ci, blocks);
+    check_decode_blocks(ci, blocks);
 }
 
 }  // namespace j2l

@@ -7,11 +7,12 @@
 #include <string>
 #include <vector>
 
-#include "common.hpp"
+#include "types.hpp"
 
 namespace j2l {
 
-enum ColorSpace : int { GRAY = 0, SRGB = 1, RGB = 2, YUV = 3, YCBCR = 4 };
+// ColorSpace in the order of src/model.rs:1-12.
+enum ColorSpace : int { GRAY = 0, RGB = 1, SRGB = 2, YUV = 3, YCBCR = 4 };
 
 struct SubbandQuant {
     uint8_t res, band, exponent;
@@ -109,5 +110,11 @@ void parse_codestream(const uint8_t* b, size_t n, CodestreamInfo& ci);
 // decode/t2.rs:75-256: walk the LRCP packets; fills blocks (component-major
 // band order, raster within band).
 void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks);
+// decode/t1.rs:96-127 per-block checks (zero bit-planes, pass counts).
+void check_decode_blocks(const CodestreamInfo& ci, const std::vector<DecBlock>& blocks);
+// Every host-side step of decode_jp2 before Tier-1 (decode/mod.rs:56-129):
+// JP2 boxes, codestream markers, header consistency, packets, block checks.
+// Throws Error(DECODE_FAILED / UNSUPPORTED_FEATURE) on malformed input.
+void parse_for_decode(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamInfo& ci, std::vector<DecBlock>& blocks);
 
 }  // namespace j2l

@@ -29,7 +29,7 @@
 /* band orientation (plan/mod.rs:45-50; tier1/helpers.rs:13-20) */
 enum { ORA_LL = 0, ORA_HL = 1, ORA_LH = 2, ORA_HH = 3 };
 /* colour spaces (model.rs:1-12) */
-enum { ORA_GRAY = 0, ORA_SRGB = 1, ORA_RGB = 2, ORA_YUV = 3, ORA_YCBCR = 4 };
+enum { ORA_GRAY = 0, ORA_RGB = 1, ORA_SRGB = 2, ORA_YUV = 3, ORA_YCBCR = 4 }; /* model.rs:1-12 order */
 /* pass kinds (t1.rs:99-104) */
 enum { ORA_SP = 0, ORA_MR = 1, ORA_CL = 2 };
 

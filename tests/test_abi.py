@@ -98,3 +98,31 @@ def test_error_classes():
     assert u.is_decode_failure() and u.is_unsupported_feature()
     assert j.EncodeOptions().quality == 85 and j.EncodeOptions().format == j.OutputFormat.JP2
     assert j.EncodeOptions.from_preset(j.Preset.DOCUMENT_LOW).quality == 30
+
+
+def test_colorspace_numbering_follows_model_rs():
+    """ColorSpace discriminants in src/model.rs:1-12 declaration order, in
+    the header, the Python enum and the library's own view validation."""
+    text = HEADER.read_text()
+    vals = dict(re.findall(r"JP2LAM_(GRAY|RGB|SRGB|YUV|YCBCR) = (\d)", text))
+    assert vals == {"GRAY": "0", "RGB": "1", "SRGB": "2", "YUV": "3", "YCBCR": "4"}
+    assert [c.name for c in sorted(j.ColorSpace)] == ["GRAY", "RGB", "SRGB", "YUV", "YCBCR"]
+    comp = j.Component(np.zeros(4, np.int32), 2, 2)
+    # a gray view with three components is rejected with the reference's
+    # name for colour space 0; colour space 2 (Srgb) with one component too
+    with pytest.raises(j.InvalidInput, match="Gray images must have exactly 1"):
+        j.encode(j.Image(2, 2, [comp] * 3, j.ColorSpace.GRAY))
+    with pytest.raises(j.InvalidInput, match="Srgb images must have exactly 3"):
+        j.encode(j.Image(2, 2, [comp], j.ColorSpace.SRGB))
+    with pytest.raises(j.InvalidInput, match="Rgb images must have exactly 3"):
+        j.encode(j.Image(2, 2, [comp], j.ColorSpace.RGB))
+
+
+def test_batch_profile_getters_without_device(lib):
+    enc = j.BatchEncoder(j.EncodeOptions(50, j.OutputFormat.J2K))
+    assert enc.profile() is None and not enc.has_profile
+    dec = j.BatchDecoder()
+    assert dec.profile() is None
+    p = _lib.BatchProfileC()
+    assert lib.jp2lam_batch_encoder_profile(None, C.byref(p), None, 0) == -1
+    assert lib.jp2lam_batch_decoder_profile(None, C.byref(p), None, 0) == -1

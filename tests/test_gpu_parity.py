@@ -299,11 +299,22 @@ def test_crate_api_roundtrip(gpu):
     with pytest.raises(j.InvalidInput, match="profile mismatch"):
         be.encode_one(j.Image.from_gray_bytes(32, 40, px[:, :32].tobytes()))
     assert be.encoded_count == 1
+    prof = be.profile()
+    assert prof == j.BatchProfile(64, 40, j.ColorSpace.GRAY, (j.BatchComponentProfile(64, 40, 8, False, 1, 1),),
+                                  85, j.OutputFormat.JP2)
     bd = j.BatchDecoder()
+    assert bd.profile() is None
     bd.decode_one(data)
     with pytest.raises(j.InvalidInput, match="profile mismatch"):
         bd.decode_one(j.encode_u8(px[:, :32], 85))
     assert bd.decoded_count == 1
+    # BatchProfile::for_decode: quality 0, JP2 (batch.rs:152-154)
+    assert bd.profile() == j.BatchProfile(64, 40, j.ColorSpace.GRAY,
+                                          (j.BatchComponentProfile(64, 40, 8, False, 1, 1),), 0, j.OutputFormat.JP2)
+    bd3 = j.BatchDecoder()
+    bd3.decode_one(j.encode_u8(synth.photo_page(30, 20, seed=1), 75))
+    p3 = bd3.profile()
+    assert p3.colorspace == j.ColorSpace.SRGB and len(p3.components) == 3
     rgb = synth.photo_page(30, 20, seed=1)
     im3 = j.Image.from_rgb_bytes(30, 20, rgb.tobytes())
     assert j.encode(im3, j.EncodeOptions(75)) == j.encode_u8(rgb, 75)
