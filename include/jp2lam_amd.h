@@ -256,26 +256,25 @@ void jp2g_last_block_passes(int index, int32_t* out);
 void jp2g_last_block_bytes(int index, uint8_t* out);
 double jp2g_last_lambda(int group);
 /* Tier-1 encoder selection (no reference counterpart; the reference codes
- * every block the same way, t1.rs:287-290): mode 0 = tuned: the first batch
- * of each size class (bit length of its code-block count) from
- * `lane_min_blocks` up runs the coders below (the hybrid at split points
- * 0.55 and 0.75) and the fastest is kept for the class, smaller batches
- * use one wavefront per code-block; the tuned choices are shared by every
- * engine in the process (serving slots included), and a batch of a class
- * that another engine is probing runs the hybrid; 1 = one wavefront per
- * code-block; 2 = one lane per code-block; 3 = one lane per code-block
- * from `lane_min_blocks` code-blocks up; 4 = hybrid: blocks whose formed
- * symbol stream is longer than 0.55 x the batch's longest are coded by a
- * wavefront each, the rest by lanes, in one launch.  Setting the mode
- * clears the tuned choices.  Output is the same in every mode. */
+ * every block the same way, t1.rs:287-290): mode 0 = tuned: the coder and
+ * split point come from a table that jp2lam_amd/autotune.py measured
+ * offline on an idle MI355X, looked up by the batch's size class (bit
+ * length of its code-block count); nothing is probed or timed while
+ * encoding, so a batch's cost does not depend on earlier calls; 1 = one
+ * wavefront per code-block; 2 = one lane per code-block; 3 = one lane per
+ * code-block from `lane_min_blocks` code-blocks up; 4 = hybrid: blocks whose
+ * formed symbol stream is longer than the split fraction (see
+ * jp2g_set_hybrid_frac) x the batch's longest are coded by a wavefront
+ * each, the rest by lanes, in one launch.  Output is the same in every
+ * mode. */
 void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks);
 /* 1 when the last encode ran the lane-per-block coder. */
 int jp2g_last_t1_lanes(void);
 /* Tier-1 coder of the last encode: 1 wave per block, 2 lane per block,
  * 4 hybrid (long streams on the scalar coder, short ones on lanes). */
 int jp2g_last_t1_coder(void);
-/* Hybrid split point: streams longer than frac x the batch's longest go to
- * the scalar coder (0 < frac < 1; default 0.55).  Clears the tuned choices. */
+/* Hybrid split point for mode 4: streams longer than frac x the batch's
+ * longest go to the scalar coder (0 < frac < 1; default 0.55). */
 void jp2g_set_hybrid_frac(float frac);
 /* Stage timings of the last call in ms: {dwt, tier1, pcrd+pack, host tier2}. */
 void jp2g_last_timings(double* out4);

@@ -759,7 +759,6 @@ void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks) {
     Engine& e = engine();
     std::lock_guard<std::mutex> g(e.mu);
     e.t1_mode = mode < 0 || mode > 4 ? 0 : mode;
-    Engine::clear_t1_tuning();
     if (lane_min_blocks) e.lane_min_blocks = lane_min_blocks;
 }
 int jp2g_last_t1_lanes(void) { return engine().last_t1_lanes ? 1 : 0; }
@@ -768,7 +767,6 @@ void jp2g_set_hybrid_frac(float frac) {
     Engine& e = engine();
     std::lock_guard<std::mutex> g(e.mu);
     e.hybrid_frac = frac > 0.0f && frac < 1.0f ? frac : 0.55f;
-    Engine::clear_t1_tuning();
 }
 int jp2g_last_block_count(void) { return (int)engine().last_blocks().size(); }
 void jp2g_last_block_info(int i, int32_t* info) {

@@ -120,15 +120,22 @@ Engine::Engine() {
     J2L_HIP(hipMemcpy(tables_, t.data(), sizeof(uint32_t) * t.size(), hipMemcpyHostToDevice));
 }
 
-Engine::T1Tuning& Engine::t1_tuning() {
-    static T1Tuning t;
-    return t;
-}
-void Engine::clear_t1_tuning() {
-    T1Tuning& t = t1_tuning();
-    std::lock_guard<std::mutex> g(t.mu);
-    t.chosen.clear();
-    t.probing.clear();
+namespace {
+enum T1Coder : int { kT1Wave = 1, kT1Lanes = 2, kT1Hybrid = 4 };
+struct T1TableEntry {
+    int bucket;  // bit length of the batch's code-block count
+    int coder;
+    float frac;
+};
+#include "t1_tuning.inc"
+}  // namespace
+
+Engine::T1Choice Engine::tuned_t1_choice(int nblocks) {
+    const int bucket = nblocks > 0 ? 32 - __builtin_clz((unsigned)nblocks) : 0;
+    T1Choice c{kT1Wave, 0.55f};
+    for (const auto& e : kT1Table)
+        if (e.bucket <= bucket) c = {e.coder, e.frac};
+    return c;
 }
 
 Engine::~Engine() {
@@ -300,37 +307,22 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     const size_t o_offs = L.add(4 * nb);
     const size_t o_packed = L.add(pack_total);
     const size_t o_sched = L.add(4 * t1_sched_words(nb));
-    // Lane-per-block Tier-1 (large batches): symbol streams, bounded per
-    // block by the band's Mb + 2 bit-planes; an overflow falls back to the
-    // wave-per-block kernel below.
-    // Tier-1 coder for this batch: forced (1 wave / 2 lane per block), a
-    // fixed threshold (3), or tuned (0): the first batch of each size class
-    // (bit length of the block count) from lane_min_blocks up runs every
-    // coder on the same coefficients and keeps the fastest; smaller batches
-    // use the wave coder.  (Classes keyed on the image count as well probed
-    // far more often in serving, where each probe stalls a batch.)
-    const int bucket = nb ? 32 - __builtin_clz((unsigned)nb) : 0;
+    // Lane-per-block and hybrid Tier-1 (large batches) code formed symbol
+    // streams, bounded per block by the band's Mb + 2 bit-planes; an
+    // overflow falls back to the wave-per-block kernel below.  The coder is
+    // forced (t1_mode 1 / 2 / 4), a fixed threshold (3), or taken from the
+    // offline-tuned table by the batch's size class (0).  Nothing is timed
+    // or probed here, so a batch's cost never depends on what ran before.
     int coder = t1_mode == 1 || t1_mode == 2 || t1_mode == 4 ? t1_mode : 1;
     float frac = hybrid_frac;
-    bool probe = false;
     if (t1_mode == 3) {
         coder = (uint32_t)nb >= lane_min_blocks ? 2 : 1;
-    } else if (t1_mode == 0 && (uint32_t)nb >= lane_min_blocks) {
-        T1Tuning& tt = t1_tuning();
-        std::lock_guard<std::mutex> g(tt.mu);
-        auto it = tt.chosen.find(bucket);
-        if (it != tt.chosen.end()) {
-            coder = it->second.coder;
-            frac = it->second.frac;
-        } else if (!tt.probing[bucket]) {
-            tt.probing[bucket] = true;
-            probe = true;
-        } else {
-            coder = 4;
-        }
+    } else if (t1_mode == 0) {
+        const T1Choice c = tuned_t1_choice(nb);
+        coder = c.coder;
+        frac = c.frac;
     }
-    static constexpr float kProbeFrac[2] = {0.55f, 0.75f};
-    const bool lanes = coder != 1 || probe;  // lane (2) and hybrid (4) code formed streams
+    const bool lanes = coder != 1;  // lane (2) and hybrid (4) code formed streams
     size_t o_pend = 0, o_soff = 0, o_pused = 0, o_pool = 0, o_lists = 0;
     unsigned long long pool_cap = 0;
     if (lanes) {
@@ -568,25 +560,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                              (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
                              (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), (uint32_t*)(D + o_lists), split, st);
     };
-    int run = probe ? 2 : coder;
-    // From 4 x lane_min_blocks up the wave coder has never been the fastest
-    // (it is the slowest by far there), so the probe skips it.
-    const bool probe_wave = probe && (uint64_t)nb < 4ull * lane_min_blocks;
-    if (probe) {
-        // the wave coder and the hybrid at both split points are timed
-        // first; the lane coder then runs below and its (identical) results
-        // are the ones used
-        J2L_HIP(hipEventRecord(ev.e[4], st));
-        if (probe_wave)
-            launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots),
-                             (BlockResult*)(D + o_res), (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns),
-                             (uint32_t*)(D + o_sched), st);
-        J2L_HIP(hipEventRecord(ev.e[5], st));
-        run_streams(4, kProbeFrac[0]);
-        J2L_HIP(hipEventRecord(ev.e[6], st));
-        run_streams(4, kProbeFrac[1]);
-        J2L_HIP(hipEventRecord(ev.e[7], st));
-    }
+    int run = coder;
 t1_again:
     J2L_HIP(hipEventRecord(ev.e[1], st));
     if (run != 1) {
@@ -646,21 +620,6 @@ t1_again:
             run = 1;
             goto t1_again;
         }
-    }
-    if (probe) {
-        const float t_wave = probe_wave ? ev.ms(4, 5) : 1e30f, t_h0 = ev.ms(5, 6), t_h1 = ev.ms(6, 7),
-                    t_lane = ev.ms(1, 2);
-        T1Choice best{1, hybrid_frac};
-        if (run != 1) {  // the streams fitted the pool
-            float t = t_wave;
-            if (t_lane < t) t = t_lane, best = {2, hybrid_frac};
-            if (t_h0 < t) t = t_h0, best = {4, kProbeFrac[0]};
-            if (t_h1 < t) t = t_h1, best = {4, kProbeFrac[1]};
-        }
-        T1Tuning& tt = t1_tuning();
-        std::lock_guard<std::mutex> g(tt.mu);
-        tt.chosen[bucket] = best;
-        tt.probing[bucket] = false;
     }
     last_t1_lanes = run == 2;
     last_t1_coder = run;

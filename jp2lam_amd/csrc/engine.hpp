@@ -90,9 +90,8 @@ public:
     const std::vector<uint32_t>& last_segment_offsets() const { return last_seg_offs_; }
     const std::vector<double>& last_lambdas() const { return last_lambdas_; }
     bool keep_debug = false;
-    // Tier-1 encoder choice: 0 = tuned per batch-size class (classes from
-    // lane_min_blocks code-blocks up time the coders once), 1 = wave per
-    // block, 2 = lane per block, 3 = lane per block from lane_min_blocks up,
+    // Tier-1 encoder choice: 0 = the offline-tuned table by batch-size class
+    // (tuned_t1_choice), 1 = wave per block, 2 = lane per block, 3 = lane per block from lane_min_blocks up,
     // 4 = hybrid (streams longer than hybrid_frac x the longest on the
     // scalar coder, the rest on lanes).
     int t1_mode = 0;
@@ -100,22 +99,14 @@ public:
     float hybrid_frac = 0.55f;
     bool last_t1_lanes = false;
     int last_t1_coder = 1;
-    // size class -> coder (1 wave, 2 lane, 4 hybrid) and the hybrid's split
-    // point; the tuner tries the hybrid at two split points.  The table is
-    // process-wide (the fastest coder is a property of the device, and every
-    // serving slot has its own engine): a size class is probed once, and
-    // batches of a class another engine is probing run the hybrid.
+    // Coder (1 wave, 2 lane, 4 hybrid) and hybrid split point for a batch of
+    // `nblocks` code-blocks in mode 0: the entry of the offline-tuned table
+    // (jp2lam_amd/autotune.py) for the batch's size class.
     struct T1Choice {
         int coder;
         float frac;
     };
-    struct T1Tuning {
-        std::mutex mu;
-        std::map<int, T1Choice> chosen;
-        std::map<int, bool> probing;
-    };
-    static T1Tuning& t1_tuning();
-    static void clear_t1_tuning();
+    static T1Choice tuned_t1_choice(int nblocks);
     // Stage timings of the last call (ms, from HIP events): prepare+DWT,
     // Tier-1, PCRD+pack, host Tier-2.
     double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;

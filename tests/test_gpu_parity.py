@@ -312,7 +312,7 @@ def test_crate_api_roundtrip(gpu):
     assert bd.profile() == j.BatchProfile(64, 40, j.ColorSpace.GRAY,
                                           (j.BatchComponentProfile(64, 40, 8, False, 1, 1),), 0, j.OutputFormat.JP2)
     bd3 = j.BatchDecoder()
-    bd3.decode_one(j.encode_u8(synth.photo_page(30, 20, seed=1), 75))
+    bd3.decode_one(j.encode_u8(synth.photo_page(64, 48, seed=1), 85))
     p3 = bd3.profile()
     assert p3.colorspace == j.ColorSpace.SRGB and len(p3.components) == 3
     rgb = synth.photo_page(30, 20, seed=1)
@@ -429,21 +429,34 @@ def test_threshold_mode_batch_crosses_threshold(gpu, ora, lib):
         lib.jp2g_set_t1_mode(0, 8192)
 
 
-def test_tuned_mode_probes_then_reuses(gpu, ora, lib):
-    """Tuned mode: the first batch of a size class runs the three coders
-    (output unchanged), later batches of the class reuse the choice."""
-    pages = list(synth.mixed_pages(5, scale=0.12))
-    singles = [ora.encode(px, q) for px, q in pages]
-    lib.jp2g_set_t1_mode(0, 50)
-    try:
-        first = j.encode_many(pages)
-        assert first == singles
-        assert lib.jp2g_last_t1_coder() in (1, 2)  # the probe's own run: lanes, or the wave coder
+def _tuned_table():
+    """(bucket, coder) entries of the compiled-in tuning table."""
+    import re
+    from pathlib import Path
+
+    text = (Path(j.__file__).resolve().parent / "csrc" / "t1_tuning.inc").read_text()
+    ids = {"kT1Wave": 1, "kT1Lanes": 2, "kT1Hybrid": 4}
+    return [(int(b), ids[c]) for b, c in re.findall(r"\{(\d+), (kT1\w+), [0-9.]+f\}", text)]
+
+
+def test_tuned_mode_follows_table(gpu, ora, lib):
+    """Tuned mode takes the coder from the offline table by the batch's
+    block-count class, the same on every call, and the bytes equal the
+    single-image encodes."""
+    from jp2lam_amd.autotune import _blocks_of
+
+    table = _tuned_table()
+    assert table and table == sorted(table)
+    docs = [(synth.document_page(2550, 3300, seed=40 + k), 85) for k in range(6)]
+    singles = [j.encode_u8(px, q) for px, q in docs]
+    assert singles[0] == ora.encode(*docs[0])
+    lib.jp2g_set_t1_mode(0, 8192)
+    for k in (1, 2, 6):
+        nb = sum(_blocks_of(px, q) for px, q in docs[:k])
+        want = 1
+        for b, c in table:
+            if b <= nb.bit_length():
+                want = c
         for _ in range(2):
-            assert j.encode_many(pages) == singles
-            chosen = lib.jp2g_last_t1_coder()
-            assert chosen in (1, 2, 4)
-        assert j.encode_many(pages) == singles
-        assert lib.jp2g_last_t1_coder() == chosen
-    finally:
-        lib.jp2g_set_t1_mode(0, 8192)
+            assert j.encode_many(docs[:k]) == singles[:k]
+            assert lib.jp2g_last_t1_coder() == want, (k, nb)
