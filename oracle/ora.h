@@ -73,6 +73,12 @@ typedef struct {
     uint8_t *bytes;            /* single terminated MQ segment */
     int nbytes;                /* full segment length */
     int body_len;              /* bytes emitted into the packet body */
+    /* PCRD record of the last lossy encode (kept for the tests): the raw
+     * curve, origin first (rate.rs:128-180), and the hull's pass counts. */
+    int raw_n, hull_n;
+    uint32_t raw_bytes[ORA_MAX_PASSES + 1];
+    double raw_dist[ORA_MAX_PASSES + 1];
+    int hull_passes[ORA_MAX_PASSES + 1];
 } ora_block;
 
 typedef struct {
@@ -97,6 +103,8 @@ void ora_irreversible_expounded_quant(uint32_t prec, uint8_t nres, uint8_t res, 
                                       uint8_t *exp, uint16_t *mant);
 double ora_get_norm_97(uint32_t level, int band);
 float ora_quant_step_f32(uint32_t precision, int band, uint8_t exponent, uint16_t mantissa);
+double ora_quality_step_scaler(int quality);
+void ora_scale_one_quant(uint8_t exponent, uint16_t mantissa, double scale, uint8_t *exp_out, uint16_t *mant_out);
 
 /* colour + DWT + quantisation */
 void ora_level_shift_f32(const int32_t *in, size_t n, float *out);
@@ -143,7 +151,39 @@ int ora_t1_encode_block(const int32_t *coeffs, int w, int h, int band, int mb, o
 int ora_t1_decode_block(int w, int h, int band, int mb, int zbp, int passes,
                         const uint8_t *data, size_t len, int32_t *out, char *err);
 
-/* PCRD */
+/* PCRD (dwt/pcrd.rs) */
+typedef struct {
+    int passes;
+    uint32_t bytes;
+    double dist;   /* cumulative distortion reduction */
+    double slope;
+} ora_rd_point;
+/* RawPassRecord (pcrd.rs:33-50) */
+typedef struct {
+    int pass_index;
+    uint32_t bytes, cumulative_bytes;
+    double distortion_delta;
+} ora_raw_record;
+/* PcrdError variants the path can raise (pcrd.rs:872-935) */
+enum {
+    ORA_PCRD_OK = 0,
+    ORA_PCRD_NONMONOTONE_CUMULATIVE_BYTES = 1,
+    ORA_PCRD_INCONSISTENT_INCREMENTAL_BYTES = 2,
+    ORA_PCRD_INVALID_DISTORTION_DELTA = 3,
+    ORA_PCRD_INVALID_ORIGIN = 4,
+    ORA_PCRD_NONMONOTONE_PASSES = 5,
+    ORA_PCRD_NONMONOTONE_POINT_BYTES = 6,
+    ORA_PCRD_NONMONOTONE_DISTORTION = 7,
+    ORA_PCRD_EMPTY_CURVE = 8,
+};
+int ora_pcrd_raw_curve(const ora_raw_record *recs, int n, ora_rd_point *out);
+int ora_pcrd_prune_hull(const ora_rd_point *raw, int n, ora_rd_point *out, int *m);
+int ora_pcrd_choose_point(const ora_rd_point *pts, int n, double lambda, int *index);
+double ora_pass_delta_pass_kind_aware(int kind, int bitplane, int newsig, int refine, double step, double sw,
+                                      int band, int quality, int cls);
+double ora_pass_delta_baseline(int bitplane, int newsig, int refine, double step, double sw);
+double ora_quality_to_lambda(int quality, uint32_t pixels);
+double ora_calibrate_lambda(double max_slope, double raw_lambda, int quality, uint32_t pixels);
 int ora_pcrd_apply(ora_encoding *enc, const ora_plan *plan);
 
 /* codec */

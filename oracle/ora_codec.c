@@ -548,6 +548,21 @@ void ora_enc_block_passes(const ora_encoding *e, int i, int32_t *out) {
         out[5 * k + 4] = b->full_lengths[k];
     }
 }
+/* PCRD record of block i (lossy encodes): the raw curve, origin first, as
+ * (bytes, cumulative distortion) pairs, and the hull's pass counts.
+ * Returns raw_n; *hull_n receives the hull length. */
+int ora_enc_block_rd(const ora_encoding *e, int i, uint32_t *raw_bytes, double *raw_dist, int32_t *hull_passes,
+                     int32_t *hull_n) {
+    const ora_block *b = &e->blocks[i];
+    for (int k = 0; k < b->raw_n; k++) {
+        raw_bytes[k] = b->raw_bytes[k];
+        raw_dist[k] = b->raw_dist[k];
+    }
+    for (int k = 0; k < b->hull_n; k++) hull_passes[k] = b->hull_passes[k];
+    *hull_n = b->hull_n;
+    return b->raw_n;
+}
+double ora_enc_max_slope(const ora_encoding *e, int c) { return e->max_slope[c]; }
 void ora_enc_block_bytes(const ora_encoding *e, int i, uint8_t *out) {
     const ora_block *b = &e->blocks[i];
     if (b->nbytes) memcpy(out, b->bytes, (size_t)b->nbytes);

@@ -10,19 +10,89 @@
 
 #include "ora.h"
 
-typedef struct {
-    int passes;
-    uint32_t bytes;
-    double dist;
-    double slope;
-} pt;
+typedef ora_rd_point pt;
 
-/* pcrd.rs:541-554 */
+/* pcrd.rs:541-554 (the NegativeByteDelta case cannot arise on validated
+ * curves, whose bytes never decrease) */
 static double rd_slope(const pt *a, const pt *b) {
     double db = (double)(b->bytes - a->bytes);
     double dd = b->dist - a->dist;
     if (db == 0.0) return dd > 0.0 ? INFINITY : 0.0;
     return dd / db;
+}
+
+/* pcrd.rs:527-539 */
+static void fill_slopes(pt *p, int n) {
+    if (n <= 0) return;
+    p[0].slope = INFINITY;
+    for (int i = 1; i < n; i++) p[i].slope = rd_slope(&p[i - 1], &p[i]);
+}
+
+/* pcrd.rs:151-202: origin, then one point per pass record, with the
+ * record validation.  out holds n + 1 points. */
+int ora_pcrd_raw_curve(const ora_raw_record *recs, int n, pt *out) {
+    out[0] = (pt){0, 0, 0.0, INFINITY};
+    uint32_t prev = 0;
+    double cum = 0.0;
+    for (int i = 0; i < n; i++) {
+        const ora_raw_record *r = &recs[i];
+        if (r->cumulative_bytes < prev) return ORA_PCRD_NONMONOTONE_CUMULATIVE_BYTES;
+        if (r->bytes != r->cumulative_bytes - prev) return ORA_PCRD_INCONSISTENT_INCREMENTAL_BYTES;
+        if (!isfinite(r->distortion_delta) || r->distortion_delta < 0.0) return ORA_PCRD_INVALID_DISTORTION_DELTA;
+        cum += r->distortion_delta;
+        out[i + 1] = (pt){r->pass_index + 1, r->cumulative_bytes, cum, 0.0};
+        prev = r->cumulative_bytes;
+    }
+    fill_slopes(out, n + 1);
+    return ORA_PCRD_OK;
+}
+
+/* pcrd.rs:209-243: push each point, then drop the middle of the last three
+ * while its slopes do not strictly decrease. */
+int ora_pcrd_prune_hull(const pt *raw, int n, pt *out, int *m_out) {
+    if (n <= 0) return ORA_PCRD_EMPTY_CURVE;
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+        out[m++] = raw[i];
+        while (m >= 3) {
+            double sab = rd_slope(&out[m - 3], &out[m - 2]);
+            double sbc = rd_slope(&out[m - 2], &out[m - 1]);
+            if (sbc >= sab) {
+                out[m - 2] = out[m - 1];
+                m--;
+            } else {
+                break;
+            }
+        }
+    }
+    fill_slopes(out, m);
+    *m_out = m;
+    return ORA_PCRD_OK;
+}
+
+/* pcrd.rs:488-525 */
+static int validate_curve(const pt *p, int n) {
+    if (n <= 0) return ORA_PCRD_EMPTY_CURVE;
+    if (p[0].passes != 0 || p[0].bytes != 0 || p[0].dist != 0.0) return ORA_PCRD_INVALID_ORIGIN;
+    for (int i = 1; i < n; i++) {
+        if (p[i].passes <= p[i - 1].passes) return ORA_PCRD_NONMONOTONE_PASSES;
+        if (p[i].bytes < p[i - 1].bytes) return ORA_PCRD_NONMONOTONE_POINT_BYTES;
+        if (p[i].dist < p[i - 1].dist) return ORA_PCRD_NONMONOTONE_DISTORTION;
+    }
+    return ORA_PCRD_OK;
+}
+
+/* pcrd.rs:406-424: the deepest point reached while slopes stay >= lambda */
+int ora_pcrd_choose_point(const pt *p, int n, double lambda, int *index) {
+    int rc = validate_curve(p, n);
+    if (rc) return rc;
+    int best = 0;
+    for (int i = 1; i < n; i++) {
+        if (p[i].slope >= lambda) best = i;
+        else break;
+    }
+    *index = best;
+    return ORA_PCRD_OK;
 }
 
 /* pcrd.rs:728-748 */
@@ -60,9 +130,19 @@ static double masking(double raw, int cls, int q) {
     return raw * blended;
 }
 
+/* pcrd.rs:652-658 */
+double ora_pass_delta_baseline(int bitplane, int newsig, int refine, double step, double sw) {
+    int bp = bitplane < 30 ? bitplane : 30;
+    double pw = (double)(1ull << bp) * step;
+    double energy = pw * pw;
+    double sig = (double)newsig * energy;
+    double ref = (double)refine * energy * 0.25;
+    return sw * (sig + ref);
+}
+
 /* pcrd.rs:669-696 */
-static double pass_delta(int kind, int bitplane, int newsig, int refine, double step,
-                         double sw, int band, int q, int cls) {
+double ora_pass_delta_pass_kind_aware(int kind, int bitplane, int newsig, int refine, double step,
+                                      double sw, int band, int q, int cls) {
     int bp = bitplane < 30 ? bitplane : 30;
     double pw = (double)(1ull << bp) * step;
     double energy = pw * pw;
@@ -75,7 +155,7 @@ static double pass_delta(int kind, int bitplane, int newsig, int refine, double 
 }
 
 /* pcrd.rs:1425-1456 */
-static double quality_to_lambda(int q, uint32_t pixels) {
+double ora_quality_to_lambda(int q, uint32_t pixels) {
     if (q >= 100) return 0.0;
     if (q == 0) return 1.7976931348623157e308;
     double t = ((double)q - 1.0) / 98.0;
@@ -119,43 +199,49 @@ static double component_weight(const ora_plan *p, int c) {
     return 1.20 + 1.80 * pow(1.0 - q / 99.0, 0.70);
 }
 
-/* Build the pruned hull for one block (rate.rs:128-180, pcrd.rs:151-252).
- * Returns the number of hull points in h[]. */
-static int build_hull(const ora_block *b, const ora_plan *p, double cw, pt *h) {
+/* pcrd.rs:1493-1514: scale the quality's lambda down when even q=1's lambda
+ * lies above 1.2 x the steepest finite positive slope. */
+double ora_calibrate_lambda(double max_slope, double raw_lambda, int quality, uint32_t pixels) {
+    (void)quality;
+    if (max_slope <= 0.0) return raw_lambda;
+    double floor_l = ora_quality_to_lambda(1, pixels);
+    double target = max_slope * 1.2;
+    if (target >= floor_l) return raw_lambda;
+    return raw_lambda * (target / floor_l);
+}
+
+/* Raw pass records of one block (rate.rs:128-180), its raw curve and the
+ * pruned hull (pcrd.rs:246-252).  The raw curve is kept in the block for the
+ * tests.  Returns the number of hull points in h[]. */
+static int build_hull(ora_block *b, const ora_plan *p, double cw, pt *h) {
     const ora_sq *sq = NULL;
     for (int i = 0; i < p->nbands; i++)
         if (p->sq[i].resolution == b->res && p->sq[i].band == b->band) sq = &p->sq[i];
     double step = sq ? quant_step_f64(sq) : 1.0;
     double sw = subband_weight(p->levels + 1, b->res, b->band);
-    pt raw[ORA_MAX_PASSES + 1];
-    raw[0] = (pt){0, 0, 0.0, INFINITY};
-    double cum = 0.0;
+    ora_raw_record recs[ORA_MAX_PASSES];
+    uint32_t prev = 0;
     for (int i = 0; i < b->npasses; i++) {
         const ora_pass *ps = &b->passes[i];
         int refine = ps->kind == ORA_MR ? ps->significant_before : 0;
-        double dd = pass_delta(ps->kind, ps->bitplane, ps->newly_significant, refine, step, sw,
-                               b->band, p->quality, b->cls) * cw;
-        cum += dd;
-        raw[i + 1] = (pt){i + 1, (uint32_t)ps->cumulative_length + 2u, cum, 0.0};
+        double dd = ora_pass_delta_pass_kind_aware(ps->kind, ps->bitplane, ps->newly_significant, refine, step, sw,
+                                                   b->band, p->quality, b->cls) * cw;
+        uint32_t cum = (uint32_t)ps->cumulative_length + 2u; /* HEADER_OVERHEAD_BYTES, rate.rs:31 */
+        recs[i] = (ora_raw_record){i, cum - prev, cum, dd};
+        prev = cum;
     }
-    int n = b->npasses + 1;
-    for (int i = 1; i < n; i++) raw[i].slope = rd_slope(&raw[i - 1], &raw[i]);
-    int m = 0;
+    pt raw[ORA_MAX_PASSES + 1];
+    int n = b->npasses + 1, m = 0;
+    /* the encoder's records always validate; a failure here is a bug */
+    if (ora_pcrd_raw_curve(recs, b->npasses, raw) != ORA_PCRD_OK) abort();
+    ora_pcrd_prune_hull(raw, n, h, &m);
+    b->raw_n = n;
     for (int i = 0; i < n; i++) {
-        h[m++] = raw[i];
-        while (m >= 3) {
-            double sab = rd_slope(&h[m - 3], &h[m - 2]);
-            double sbc = rd_slope(&h[m - 2], &h[m - 1]);
-            if (sbc >= sab) {
-                h[m - 2] = h[m - 1];
-                m--;
-            } else {
-                break;
-            }
-        }
+        b->raw_bytes[i] = raw[i].bytes;
+        b->raw_dist[i] = raw[i].dist;
     }
-    h[0].slope = INFINITY;
-    for (int i = 1; i < m; i++) h[i].slope = rd_slope(&h[i - 1], &h[i]);
+    b->hull_n = m;
+    for (int i = 0; i < m; i++) b->hull_passes[i] = h[i].passes;
     return m;
 }
 
@@ -197,24 +283,17 @@ int ora_pcrd_apply(ora_encoding *enc, const ora_plan *p) {
             for (int j = 1; j < hn[i]; j++)
                 if (isfinite(h[j].slope) && h[j].slope > 0.0 && h[j].slope > mx) mx = h[j].slope;
         }
-        double raw = quality_to_lambda(p->quality, pixels);
-        double lambda = raw;
-        if (mx > 0.0) {
-            double floor_l = quality_to_lambda(1, pixels);
-            double target = mx * 1.2;
-            if (target < floor_l) lambda = raw * (target / floor_l);
-        }
+        double raw = ora_quality_to_lambda(p->quality, pixels);
+        double lambda = ora_calibrate_lambda(mx, raw, p->quality, pixels);
         enc->lambda[c] = lambda;
         enc->max_slope[c] = mx;
         for (int i = 0; i < enc->nblocks; i++) {
             ora_block *b = &enc->blocks[i];
             if (b->comp != c) continue;
             pt *h = hulls + (size_t)i * (ORA_MAX_PASSES + 1);
-            int keep = 0;
-            for (int j = 1; j < hn[i]; j++) {
-                if (h[j].slope >= lambda) keep = h[j].passes;
-                else break;
-            }
+            int k = 0;
+            if (ora_pcrd_choose_point(h, hn[i], lambda, &k) != ORA_PCRD_OK) abort();
+            int keep = h[k].passes;
             truncate_block(b, keep);
         }
     }

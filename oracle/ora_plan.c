@@ -66,7 +66,7 @@ uint32_t ora_max_decompositions(uint32_t w, uint32_t h) {
 }
 
 /* plan/derive.rs:74-86 */
-static double quality_step_scaler(int q) {
+double ora_quality_step_scaler(int q) {
     if (q == 0) return 8.0;
     if (q <= 50) return pow(2.0, (50.0 - (double)q) / 20.0);
     if (q <= 90) return pow(2.0, -(((double)q - 50.0) / 20.0));
@@ -88,6 +88,14 @@ static void scale_one_quant(ora_sq *sq, double scale) {
     if (m > 2047.0) m = 2047.0;
     sq->exponent = (uint8_t)e;
     sq->mantissa = (uint16_t)m;
+}
+
+/* scale_one_quant on one (exponent, mantissa) pair, for the tests */
+void ora_scale_one_quant(uint8_t exponent, uint16_t mantissa, double scale, uint8_t *exp_out, uint16_t *mant_out) {
+    ora_sq sq = {0, 0, exponent, mantissa};
+    scale_one_quant(&sq, scale);
+    *exp_out = sq.exponent;
+    *mant_out = sq.mantissa;
 }
 
 /* plan/validate.rs:4-62 plus model.rs colour-space component counts */
@@ -151,7 +159,7 @@ int ora_build_plan(const int32_t *const *planes, uint32_t w, uint32_t h, int nco
     }
     p->nbands = nb;
     if (!p->lossless) {
-        double s = quality_step_scaler(quality);
+        double s = ora_quality_step_scaler(quality);
         for (int i = 0; i < nb; i++) scale_one_quant(&p->sq[i], s);
     }
     return ORA_OK;

@@ -18,6 +18,17 @@ ORACLE_LIB = ORACLE_DIR / "build" / "liboracle.so"
 vp = C.c_void_p
 
 
+class RdPoint(C.Structure):
+    """ora_rd_point: one point of a PCRD curve (pcrd.rs PcrdPoint)."""
+    _fields_ = [("passes", C.c_int), ("bytes", C.c_uint32), ("dist", C.c_double), ("slope", C.c_double)]
+
+
+class RawRecord(C.Structure):
+    """ora_raw_record: RawPassRecord (pcrd.rs:33-50)."""
+    _fields_ = [("pass_index", C.c_int), ("bytes", C.c_uint32), ("cumulative_bytes", C.c_uint32),
+                ("distortion_delta", C.c_double)]
+
+
 def build_oracle() -> Path:
     subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
     return ORACLE_LIB
@@ -72,6 +83,19 @@ class Oracle:
             "ora_decode_recon_f32": (C.c_int, [vp, C.c_size_t, vp, C.c_size_t]),
             "ora_analyze_content_flat": (None, [vp, C.c_size_t, C.c_int, C.c_int, vp]),
             "ora_classify_content": (C.c_int, [vp]),
+            "ora_pcrd_raw_curve": (C.c_int, [vp, C.c_int, vp]),
+            "ora_pcrd_prune_hull": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_int)]),
+            "ora_pcrd_choose_point": (C.c_int, [vp, C.c_int, C.c_double, C.POINTER(C.c_int)]),
+            "ora_pass_delta_pass_kind_aware": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                            C.c_double, C.c_int, C.c_int, C.c_int]),
+            "ora_pass_delta_baseline": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
+            "ora_quality_to_lambda": (C.c_double, [C.c_int, C.c_uint32]),
+            "ora_calibrate_lambda": (C.c_double, [C.c_double, C.c_double, C.c_int, C.c_uint32]),
+            "ora_quality_step_scaler": (C.c_double, [C.c_int]),
+            "ora_scale_one_quant": (None, [C.c_uint8, C.c_uint16, C.c_double, C.POINTER(C.c_uint8),
+                                           C.POINTER(C.c_uint16)]),
+            "ora_enc_block_rd": (C.c_int, [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_int32)]),
+            "ora_enc_max_slope": (C.c_double, [vp, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -233,6 +257,35 @@ class Oracle:
         self.L.ora_mq_decode_stream(data, len(data), c.ctypes.data, len(c), out.ctypes.data)
         return out
 
+    # ---- PCRD (dwt/pcrd.rs) ------------------------------------------------
+    def raw_curve(self, records):
+        """build_raw_curve: records are (pass_index, bytes, cumulative, dD).
+        Returns (status, [RdPoint])."""
+        recs = (RawRecord * max(len(records), 1))(*[RawRecord(*r) for r in records])
+        out = (RdPoint * (len(records) + 1))()
+        rc = self.L.ora_pcrd_raw_curve(recs, len(records), out)
+        return rc, list(out) if rc == 0 else None
+
+    def hull(self, points):
+        arr = (RdPoint * len(points))(*points)
+        out = (RdPoint * len(points))()
+        m = C.c_int()
+        rc = self.L.ora_pcrd_prune_hull(arr, len(points), out, C.byref(m))
+        assert rc == 0
+        return list(out)[:m.value]
+
+    def choose(self, points, lam):
+        """choose_block_point: (status, chosen RdPoint)."""
+        arr = (RdPoint * len(points))(*points)
+        k = C.c_int(-1)
+        rc = self.L.ora_pcrd_choose_point(arr, len(points), lam, C.byref(k))
+        return rc, (points[k.value] if rc == 0 else None)
+
+    def hull_curve(self, records):
+        rc, raw = self.raw_curve(records)
+        assert rc == 0, rc
+        return self.hull(raw)
+
     def tables(self):
         qe = np.zeros(94, np.uint16)
         mps, nmps, nlps = (np.zeros(94, np.uint8) for _ in range(3))
@@ -284,3 +337,16 @@ class Encoding:
 
     def lam(self, c) -> float:
         return self.o.L.ora_enc_lambda(self.h, c)
+
+    def max_slope(self, c) -> float:
+        return self.o.L.ora_enc_max_slope(self.h, c)
+
+    def rd(self, i):
+        """(raw bytes, raw cumulative distortion, hull pass counts) of block
+        i's PCRD curve, origin first."""
+        rb = np.zeros(97, np.uint32)
+        rdist = np.zeros(97, np.float64)
+        hp = np.zeros(97, np.int32)
+        hn = C.c_int32()
+        n = self.o.L.ora_enc_block_rd(self.h, i, rb.ctypes.data, rdist.ctypes.data, hp.ctypes.data, C.byref(hn))
+        return rb[:n], rdist[:n], hp[:hn.value]
