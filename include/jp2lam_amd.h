@@ -118,6 +118,20 @@ jp2lam_status jp2lam_analyze_content(const jp2lam_image_view* image, jp2lam_cont
  * device (on_device) resident. */
 jp2lam_status jp2lam_analyze_content_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
                                         int on_device, jp2lam_content_stats* out);
+/* No counterpart in the reference.  jp2lam_analyze_content_u8 for pixels
+ * already in device memory, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * it calls no hipMalloc/hipFree and touches no engine.  Its counters live in
+ * a fixed 257 x 8-byte device array per GPU (256 luma bins + the chroma
+ * spread sum), cleared by a kernel at the start of every call; calls that
+ * overlap on one GPU share that array and race, so serialise them.  The
+ * work runs on `stream` (a hipStream_t; NULL = the default stream), which
+ * the call synchronises.  width and height are at most
+ * JP2LAM_NOALLOC_MAX_DIM (INVALID_INPUT beyond).  Results equal
+ * jp2lam_analyze_content_u8 with on_device = 1. */
+#define JP2LAM_NOALLOC_MAX_DIM 65535u
+jp2lam_status jp2lam_analyze_content_u8_noalloc(const uint8_t* dev_pixels, uint32_t width, uint32_t height,
+                                                uint32_t channels, void* stream, jp2lam_content_stats* out);
 /* Image::from_gray_bytes / from_rgb_bytes (src/model.rs:87-140) + encode:
  * interleaved u8 pixels, channels 1 (gray) or 3 (sRGB). */
 jp2lam_status jp2lam_encode_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,

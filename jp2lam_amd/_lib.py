@@ -84,6 +84,8 @@ SIGNATURES = [
     ("jp2lam_analyze_content", C.c_int, [C.POINTER(ImageViewC), C.POINTER(ContentStatsC)]),
     ("jp2lam_analyze_content_u8", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                             C.POINTER(ContentStatsC)]),
+    ("jp2lam_analyze_content_u8_noalloc", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                                    C.POINTER(ContentStatsC)]),
     ("jp2lam_encode_with_psnr", C.c_int, [C.POINTER(ImageViewC), C.POINTER(EncodeOptionsC), C.POINTER(u8p),
                                           C.POINTER(C.c_size_t), C.POINTER(MetricsC)]),
     ("jp2lam_encode_u8", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(EncodeOptionsC),

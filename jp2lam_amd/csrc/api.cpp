@@ -253,6 +253,29 @@ jp2lam_status jp2lam_analyze_content_u8(const uint8_t* pixels, uint32_t width, u
     });
 }
 
+jp2lam_status jp2lam_analyze_content_u8_noalloc(const uint8_t* dev_pixels, uint32_t width, uint32_t height,
+                                                uint32_t channels, void* stream, jp2lam_content_stats* out) {
+    return guarded([&] {
+        if (!dev_pixels || !out) return fail(INVALID_INPUT, "null argument");
+        if (channels != 1 && channels != 3) return fail(INVALID_INPUT, "channels must be 1 (gray) or 3 (sRGB)");
+        if (width > JP2LAM_NOALLOC_MAX_DIM || height > JP2LAM_NOALLOC_MAX_DIM)
+            return fail(INVALID_INPUT, "image dimensions " + std::to_string(width) + "x" + std::to_string(height) +
+                                           " exceed the no-allocation maximum of " +
+                                           std::to_string(JP2LAM_NOALLOC_MAX_DIM) + " per side");
+        std::memset(out, 0, sizeof(*out));
+        const uint64_t n = (uint64_t)width * height;
+        uint64_t hist[257] = {};
+        if (n) content_hist_noalloc(dev_pixels, n, (int)channels, (hipStream_t)stream, hist);
+        const ContentOut r = content_from_hist(hist, n);
+        out->bright_share = r.bright_share;
+        out->entropy = r.entropy;
+        out->colorfulness = r.colorfulness;
+        out->content_class = (uint32_t)r.graphics;
+        out->quality = r.graphics ? 55u : 78u;
+        return JP2LAM_OK;
+    });
+}
+
 jp2lam_status jp2lam_encode_u8(const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
                                const jp2lam_encode_options* opts, uint8_t** out, size_t* out_len) {
     return guarded([&] {

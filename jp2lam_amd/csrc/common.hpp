@@ -55,6 +55,10 @@ void launch_metric_cells(const MetricSeg* segs, int nseg, uint32_t ncells, doubl
 void launch_metric_reduce(const MetricSeg* segs, int nseg, const double* cells, double* out, hipStream_t st);
 void launch_content_hist(const void* src, uint64_t n, int channels, int planar_i32, int gray, unsigned long long* out,
                          hipStream_t st);
+// Content histogram of `n` interleaved u8 pixels in device memory into the
+// file-scope scratch array of kernels_metrics.hip (no allocation), read back
+// into host_hist[257]; synchronises `st`.
+void content_hist_noalloc(const void* dev_src, uint64_t n, int channels, hipStream_t st, uint64_t* host_hist);
 void launch_pack(int n, int nimg, const uint32_t* img_first, const uint32_t* blk_img, const uint64_t* pack_base,
                  const BlockJob* jobs, const Selection* sel, uint32_t* offs, uint32_t* totals, const uint8_t* slots,
                  uint8_t* packed, hipStream_t st, bool offsets_only);

@@ -848,21 +848,7 @@ ContentOut Engine::analyze_content(const void* src, int src_kind, size_t n, bool
                         (unsigned long long*)(D + o_hist), st);
     J2L_HIP(hipMemcpyAsync(pin_.base, D + o_hist, 8 * 257, hipMemcpyDeviceToHost, st));
     J2L_HIP(hipStreamSynchronize(st));
-    const uint64_t* h = (const uint64_t*)pin_.base;
-    const float total = (float)(uint64_t)n;
-    uint64_t bright = 0;
-    for (int v = 225; v < 256; v++) bright += h[v];  // luma > 224
-    float entropy = 0.0f;
-    for (int v = 0; v < 256; v++)
-        if (h[v]) {
-            const float p = (float)h[v] / total;
-            entropy -= p * std::log2(p);
-        }
-    o.bright_share = (float)bright / total;
-    o.entropy = entropy;
-    o.colorfulness = (float)h[256] / total;
-    // content.rs:96-105
-    o.graphics = entropy < 3.0f || (o.bright_share > 0.45f && entropy < 5.0f);
+    o = content_from_hist((const uint64_t*)pin_.base, n);
     return o;
 }
 

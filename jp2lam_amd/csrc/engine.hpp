@@ -51,10 +51,6 @@ struct DecOut {
 };
 
 // ContentStats and ContentClass (content.rs:80-105).
-struct ContentOut {
-    float bright_share = 0.0f, entropy = 0.0f, colorfulness = 0.0f;
-    int graphics = 1;  // 1 Graphics, 0 Photo (all-zero statistics classify as Graphics)
-};
 
 // Tier-1 block statistics of the last encode (tests and counters).
 struct BlockStat {

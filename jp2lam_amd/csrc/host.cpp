@@ -957,4 +957,25 @@ void parse_for_decode(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamI
     check_decode_blocks(ci, blocks);
 }
 
+// content.rs:120-174: f32 statistics from the 256 luma bins and the
+// chroma-spread sum, then ContentStats::classify (content.rs:96-105).
+ContentOut content_from_hist(const uint64_t* h, uint64_t n) {
+    ContentOut o;
+    if (n == 0) return o;  // content.rs:155-157
+    const float total = (float)n;
+    uint64_t bright = 0;
+    for (int v = 225; v < 256; v++) bright += h[v];  // luma > 224
+    float entropy = 0.0f;
+    for (int v = 0; v < 256; v++)
+        if (h[v]) {
+            const float p = (float)h[v] / total;
+            entropy -= p * std::log2(p);
+        }
+    o.bright_share = (float)bright / total;
+    o.entropy = entropy;
+    o.colorfulness = (float)h[256] / total;
+    o.graphics = entropy < 3.0f || (o.bright_share > 0.45f && entropy < 5.0f);
+    return o;
+}
+
 }  // namespace j2l

@@ -76,6 +76,15 @@ void write_codestream_tail(std::vector<uint8_t>& out);
 // JP2 boxes before the codestream (jp2.rs:13-50).
 void write_jp2_head(const Plan& p, size_t codestream_len, std::vector<uint8_t>& out);
 
+// ---- content analysis (content.rs:46-174) ----------------------------------------
+struct ContentOut {
+    float bright_share = 0.0f, entropy = 0.0f, colorfulness = 0.0f;
+    int graphics = 1;  // 1 Graphics, 0 Photo (all-zero statistics classify as Graphics)
+};
+// Statistics and class from the device histogram: hist[0..255] luma bins,
+// hist[256] the sum of chroma spreads, n samples (content.rs:120-174).
+ContentOut content_from_hist(const uint64_t* hist, uint64_t n);
+
 // ---- decoding -----------------------------------------------------------------
 struct Jp2Info {
     uint32_t width = 0, height = 0;

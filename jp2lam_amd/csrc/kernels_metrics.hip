@@ -14,6 +14,7 @@
 // values and per-cell f64 sums follow the reference's order, so each cell's
 // SSIM equals the reference's; only the order of the image-wide sums differs.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -213,6 +214,52 @@ void launch_content_hist(const void* src, uint64_t n, int channels, int planar_i
     if (!n) return;
     const uint64_t groups = std::min<uint64_t>(std::max<uint64_t>(n / 4096, 1), 4096);
     content_hist_kernel<<<(unsigned)groups, 256, 0, st>>>(src, n, channels, planar_i32, gray, out);
+}
+
+// ---- content analysis without allocation ----------------------------------------
+// Scratch of content_hist_noalloc: the 256 luma bins and the chroma-spread
+// sum of the last call.  Each device holds its own copy for the life of the
+// process; calls that overlap on one device share it and race.
+constexpr int kContentScratchWords = 257;
+static __device__ __align__(256) unsigned long long scratch[kContentScratchWords];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxScratchDevices] = {};
+
+__global__ __launch_bounds__(256) void zero_words_kernel(unsigned long long* __restrict__ p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device.  The array's own name is a host-side stand-in and never reaches a
+// kernel or a copy.
+unsigned long long* scratch_on_current_device() {
+    int dev = 0;
+    J2L_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) throw Error(IO, "device ordinal out of range for the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        J2L_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw Error(IO, "hipGetSymbolAddress returned no address for the content scratch");
+        g_scratch_ptr[dev] = (unsigned long long*)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void content_hist_noalloc(const void* dev_src, uint64_t n, int channels, hipStream_t st, uint64_t* host_hist) {
+    unsigned long long* buf = scratch_on_current_device();
+    zero_words_kernel<<<(kContentScratchWords + 255) / 256, 256, 0, st>>>(buf, kContentScratchWords);
+    J2L_HIP(hipGetLastError());
+    launch_content_hist(dev_src, n, channels, 0, channels == 1, buf, st);
+    J2L_HIP(hipGetLastError());
+    J2L_HIP(hipMemcpyAsync(host_hist, buf, sizeof(unsigned long long) * kContentScratchWords, hipMemcpyDeviceToHost,
+                           st));
+    J2L_HIP(hipStreamSynchronize(st));
 }
 
 void launch_metric_dequant(const BlockJob* jobs, const BlockResult* res, const Selection* sel, const uint32_t* list,
