@@ -209,6 +209,13 @@ typedef struct {
 } jp2lam_decode_item;
 jp2lam_status jp2lam_encode_many(jp2lam_encode_item* items, size_t n);
 jp2lam_status jp2lam_decode_many(jp2lam_decode_item* items, size_t n);
+/* Queue form of the two calls above: the list is cut, in order, into device
+ * batches of at most `chunk_blocks` code-blocks (decodes weighed by their
+ * header size), which run on `engines` engines at once, so one batch's host
+ * work overlaps another's kernels.  0 restores the default for either value,
+ * which jp2lam_amd/autotune.py measured (at most 262144 blocks, 8 engines). */
+void jp2g_set_queue(uint32_t chunk_blocks, int engines);
+void jp2g_get_queue(uint32_t* chunk_blocks, int* engines);
 
 /* ---- serving (no counterpart in the reference) ---------------------------
  * Continuous batching over `slots` device engines: a free slot takes the

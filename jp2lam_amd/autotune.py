@@ -18,6 +18,12 @@ size class (bit length of the batch's code-block count) it keeps the
 class's workloads, and writes that table as a C++ include that the engine
 compiles in.  Output bytes are the same under every coder; the sweep checks
 that too.  Nothing is probed while requests are served.
+
+A second sweep tunes the queue form of jp2lam_encode_many /
+jp2lam_decode_many: a long mixed-page list is encoded and decoded whole
+with each (code-blocks per device batch, engines at once) pair, and the
+pair with the lowest encode + decode wall time is written to
+``t1_tuning_queue.inc``.
 """
 from __future__ import annotations
 
@@ -85,12 +91,28 @@ def main() -> None:
     ap.add_argument("--doc-counts", default="1,2,3,4,5,6,8,12,16,24,32")
     ap.add_argument("--mixed-counts", default="1,2,3,4,6,8,12,16,24")
     ap.add_argument("--mixed-seed", type=int, default=4)
+    ap.add_argument("--out-queue-inc", default=str(PKG / "csrc" / "t1_tuning_queue.inc"))
+    ap.add_argument("--queue-pages", type=int, default=64)
+    ap.add_argument("--queue-chunks", default="32768,65536,131072,262144")
+    ap.add_argument("--queue-engines", default="1,2,3")
+    ap.add_argument("--skip-coder", action="store_true", help="run the queue sweep only")
     args = ap.parse_args()
     lib = _lib.load()
     if not lib.jp2g_device_available():
         raise SystemExit("autotune needs a HIP device")
     doc_counts = [int(x) for x in args.doc_counts.split(",")]
     mixed_counts = [int(x) for x in args.mixed_counts.split(",")]
+    report = {}
+    if not args.skip_coder:
+        report.update(coder_sweep(lib, args, doc_counts, mixed_counts))
+    report["queue"] = queue_sweep(lib, args)
+    Path(args.out_json).write_text(json.dumps(report, indent=1) + "\n")
+    log(f"wrote {args.out_json}")
+
+
+def coder_sweep(lib, args, doc_counts, mixed_counts):
+    # every workload runs as one device batch on the process-wide engine
+    lib.jp2g_set_queue(1 << 18, 1)
     rows = []
     for name, pages in workloads(doc_counts, mixed_counts, args.mixed_seed):
         lib.jp2g_set_t1_mode(1, 0)
@@ -108,11 +130,52 @@ def main() -> None:
                      "t1_ms": {k: round(v, 4) for k, v in times.items()}})
         log(f"{name}: {nb} blocks, " + ", ".join(f"{k} {v:.2f}" for k, v in times.items()))
     lib.jp2g_set_t1_mode(0, 0)
+    lib.jp2g_set_queue(0, 0)
     table = choose(rows)
-    Path(args.out_json).write_text(json.dumps({"configs": [c[0] for c in CONFIGS], "rows": rows,
-                                               "table": table}, indent=1) + "\n")
     write_inc(Path(args.out_inc), table)
-    log(f"wrote {args.out_json} and {args.out_inc}")
+    log(f"wrote {args.out_inc}")
+    return {"configs": [c[0] for c in CONFIGS], "rows": rows, "table": table}
+
+
+def queue_sweep(lib, args):
+    """Whole-list encode + decode wall time of a mixed-page queue for each
+    (chunk code-blocks, engines) pair; the fastest pair is written out."""
+    pages = list(synth.mixed_pages(args.queue_pages, seed=args.mixed_seed + 1))
+    mpx = sum(p.shape[0] * p.shape[1] for p, _ in pages) / 1e6
+    log(f"queue sweep: {len(pages)} mixed pages, {mpx:.0f} Mpx")
+    lib.jp2g_set_t1_mode(0, 0)
+    lib.jp2g_set_queue(0, 0)
+    ref = j.encode_many(pages)
+    streams = [r for r in ref if not isinstance(r, Exception)]
+    j.decode_many(streams)  # grows every engine the sweep uses
+    rows = []
+    for eng in [int(x) for x in args.queue_engines.split(",")]:
+        for chunk in [int(x) for x in args.queue_chunks.split(",")]:
+            lib.jp2g_set_queue(chunk, eng)
+            best_e = best_d = math.inf
+            for _ in range(2):
+                t0 = time.perf_counter()
+                out = j.encode_many(pages)
+                t1 = time.perf_counter()
+                j.decode_many(streams)
+                t2 = time.perf_counter()
+                if out != ref:
+                    raise SystemExit(f"queue chunk {chunk} engines {eng}: output differs")
+                best_e, best_d = min(best_e, t1 - t0), min(best_d, t2 - t1)
+            rows.append({"chunk_blocks": chunk, "engines": eng, "encode_s": round(best_e, 4),
+                         "decode_s": round(best_d, 4), "encode_mpx_s": round(mpx / best_e, 1)})
+            log(f"queue chunk {chunk} engines {eng}: encode {best_e:.3f} s, decode {best_d:.3f} s")
+    lib.jp2g_set_queue(0, 0)
+    best = min(rows, key=lambda r: r["encode_s"] + r["decode_s"])
+    Path(args.out_queue_inc).write_text(
+        "// Generated by jp2lam_amd/autotune.py on an MI355X; do not edit by hand.\n"
+        "// Queue chunking for jp2lam_encode_many / jp2lam_decode_many: code-blocks\n"
+        f"// per device batch and engines running batches at once ({len(pages)} mixed\n"
+        f"// pages, {mpx:.0f} Mpx: encode {best['encode_s']:.3f} s, decode {best['decode_s']:.3f} s).\n"
+        f"constexpr uint32_t kQueueChunkBlocks = {best['chunk_blocks']};\n"
+        f"constexpr int kQueueEngines = {best['engines']};\n")
+    log(f"wrote {args.out_queue_inc}")
+    return {"pages": len(pages), "megapixels": round(mpx, 1), "rows": rows, "chosen": best}
 
 
 def _blocks_of(px, q) -> int:

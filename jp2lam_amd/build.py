@@ -22,7 +22,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "libjp2lam_amd.so"
 BUILD_DIR = PKG / "_build"
 
-SOURCES = ["kernels_dwt.hip", "kernels_t1.hip", "kernels_pcrd.hip", "kernels_metrics.hip", "host.cpp", "engine.cpp", "server.cpp", "api.cpp"]
+SOURCES = ["kernels_dwt.hip", "kernels_t1.hip", "kernels_pcrd.hip", "kernels_metrics.hip", "host.cpp", "engine.cpp", "queue.cpp", "server.cpp", "api.cpp"]
 ARCH = os.environ.get("JP2LAM_AMD_ARCH", "gfx950")
 
 

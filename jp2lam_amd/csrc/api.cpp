@@ -10,6 +10,7 @@
 #include <sstream>
 
 #include "engine.hpp"
+#include "queue.hpp"
 #include "server.hpp"
 
 using namespace j2l;
@@ -455,12 +456,7 @@ jp2lam_status jp2lam_encode_many(jp2lam_encode_item* items, size_t n) {
             v[i].src_kind = s.on_device ? SRC_DEV_U8 : SRC_HOST_U8;
             v[i].src = s.pixels;
         }
-        std::vector<EncOut> r;
-        {
-            Engine& e = engine();
-            std::lock_guard<std::mutex> lk(e.mu);
-            r = e.encode(v);
-        }
+        std::vector<EncOut> r = encode_queue(v);
         jp2lam_status first = JP2LAM_OK;
         for (size_t i = 0; i < n; i++) {
             items[i].status = r[i].status;
@@ -488,12 +484,7 @@ jp2lam_status jp2lam_decode_many(jp2lam_decode_item* items, size_t n) {
             v[i].dev_out = items[i].pixels;
             v[i].dev_cap = items[i].cap;
         }
-        std::vector<DecOut> r;
-        {
-            Engine& e = engine();
-            std::lock_guard<std::mutex> lk(e.mu);
-            r = e.decode(v);
-        }
+        std::vector<DecOut> r = decode_queue(v);
         jp2lam_status first = JP2LAM_OK;
         for (size_t i = 0; i < n; i++) {
             items[i].status = r[i].status;
@@ -786,6 +777,12 @@ void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks) {
 }
 int jp2g_last_t1_lanes(void) { return engine().last_t1_lanes ? 1 : 0; }
 int jp2g_last_t1_coder(void) { return engine().last_t1_coder; }
+void jp2g_set_queue(uint32_t chunk_blocks, int engines) { set_queue_config(chunk_blocks, engines); }
+void jp2g_get_queue(uint32_t* chunk_blocks, int* engines) {
+    const QueueConfig c = queue_config();
+    if (chunk_blocks) *chunk_blocks = c.chunk_blocks;
+    if (engines) *engines = c.engines;
+}
 void jp2g_set_hybrid_frac(float frac) {
     Engine& e = engine();
     std::lock_guard<std::mutex> g(e.mu);
