@@ -273,13 +273,9 @@ void ora_t1_luts(uint8_t zc[4][512], uint8_t sc[256], uint8_t spb[256]) {
 }
 
 static uint8_t LUT_ZC[4][512], LUT_SC[256], LUT_SPB[256];
-static int luts_ready;
-static void ensure_luts(void) {
-    if (!luts_ready) {
-        ora_t1_luts(LUT_ZC, LUT_SC, LUT_SPB);
-        luts_ready = 1;
-    }
-}
+/* Filled once when the library loads, before any thread can code a block. */
+__attribute__((constructor)) static void init_luts(void) { ora_t1_luts(LUT_ZC, LUT_SC, LUT_SPB); }
+static void ensure_luts(void) {}
 
 /* ---- flag grid (tier1/flags.rs:11-296) ----------------------------------- */
 #define F_SIG (1u << 0)
