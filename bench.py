@@ -38,9 +38,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=100.0, help="serving arrival rate, requests/s")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other workloads (512x512 q100 J2K, 4000x3000 sRGB q75, mixed-page queue)")
-    ap.add_argument("--mixed-pages", type=int, default=32,
+    ap.add_argument("--mixed-pages", type=int, default=256,
                     help="pages of the mixed-size queue leg (the full queue is 256)")
-    ap.add_argument("--mixed-batch", type=int, default=16, help="pages per device batch in the mixed-queue leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -347,70 +346,98 @@ def main():
         del dph, dph_out
 
         # mixed-size queue: 70 % documents / 30 % photos, q in {50, 75, 85, 100},
-        # encoded and decoded in device batches of --mixed-batch pages
+        # pages resident in HBM, encoded as ONE jp2lam_encode_many call and
+        # decoded as one jp2lam_decode_many call (the queue cuts itself into
+        # tuned device batches on several engines)
         if args.mixed_pages > 0:
-            note(f"mixed queue: generating {args.mixed_pages} pages")
-            host = list(synth.mixed_pages(args.mixed_pages, seed=4))
-            note("mixed queue: encode and decode in device batches")
+            workers = max(1, min(16, os.cpu_count() or 1))
+            note(f"mixed queue: generating {args.mixed_pages} pages on {workers} processes")
+            host = synth.mixed_pages_parallel(args.mixed_pages, seed=4, workers=workers)
+            note("mixed queue: one encode_many call, one decode_many call")
             devp = [torch.from_numpy(p).to(dev) for p, _ in host]
             items = [((d.data_ptr(), p.shape[0], p.shape[1], 1 if p.ndim == 2 else 3), q)
                      for d, (p, q) in zip(devp, host)]
             outs = [torch.empty_like(d) for d in devp]
-            B = max(1, args.mixed_batch)
-            chunks = [list(range(i, min(i + B, len(items)))) for i in range(0, len(items), B)]
-            # sRGB lossless pages are rejected by the reference decoder (its
-            # chroma Mb quirk), so the decode half skips them
-            dec_ids = [k for k in range(len(host)) if not (host[k][0].ndim == 3 and host[k][1] >= 100)]
-            dchunks = [dec_ids[i:i + B] for i in range(0, len(dec_ids), B)]
-            streams = [None] * len(items)
+            torch.cuda.synchronize()
 
             def enc_all():
-                for c, ch in enumerate(chunks):
-                    if c and c % 4 == 0:
-                        note(f"mixed queue: encoded {c}/{len(chunks)} batches")
-                    for k, r in zip(ch, j.encode_many([items[k] for k in ch], device=True)):
-                        if isinstance(r, Exception):
-                            raise r
-                        streams[k] = r
+                res = j.encode_many(items, device=True)
+                bad = [r for r in res if isinstance(r, Exception)]
+                if bad:
+                    raise bad[0]
+                return res
+
+            streams = enc_all()  # warm-up: grows every queue engine's arenas
+            # streams the reference decoder rejects (sRGB lossless, its chroma
+            # Mb quirk; zero-length first contributions, decode/t2.rs:243) are
+            # rejected by ours too: they stay out of the decode half and the
+            # serving leg's decode requests, and are listed with the reason
+            rejected = {}
+            for k in range(len(host)):
+                try:
+                    j.inspect_jp2(streams[k])
+                    if host[k][0].ndim == 3 and host[k][1] >= 100:
+                        raise j.DecodeFailed("sRGB lossless: chroma Mb (backend.rs:179-189)")
+                except j.Jp2LamError as e:
+                    rejected[k] = str(e)
+            dec_ids = [k for k in range(len(host)) if k not in rejected]
 
             def dec_all():
-                for c, ch in enumerate(dchunks):
-                    if c and c % 4 == 0:
-                        note(f"mixed queue: decoded {c}/{len(dchunks)} batches")
-                    for r in j.decode_many([streams[k] for k in ch],
-                                           device_outputs=[(outs[k].data_ptr(), outs[k].numel()) for k in ch]):
-                        if isinstance(r, Exception):
-                            raise r
+                res = j.decode_many([streams[k] for k in dec_ids],
+                                    device_outputs=[(outs[k].data_ptr(), outs[k].numel()) for k in dec_ids])
+                for k, r in zip(dec_ids, res):
+                    if isinstance(r, Exception):
+                        try:  # the single call carries the reason
+                            j.decode_u8(streams[k])
+                        except j.Jp2LamError as e:
+                            r = e
+                        rejected[k] = str(r)
+                return res
 
-            _, em = timed(enc_all, 1)
-            # streams the reference decoder rejects, as ours does (its
-            # zero-length first contributions, decode/t2.rs:243), stay out of
-            # the decode half and the serving leg's decode requests, and are
-            # listed with the reason
-            rejected = {}
-            for k in dec_ids:
-                try:
-                    j.decode_u8(streams[k])
-                except j.DecodeFailed as e:
-                    rejected[k] = str(e)
+            dec_all()
             dec_ids = [k for k in dec_ids if k not in rejected]
-            dchunks = [dec_ids[i:i + B] for i in range(0, len(dec_ids), B)]
-            _, dm = timed(dec_all, 1)
+            em = dm = float("inf")
+            for _ in range(2):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                again = enc_all()
+                torch.cuda.synchronize()
+                em = min(em, (time.perf_counter() - t0) * 1e3)
+                t0 = time.perf_counter()
+                dec_all()
+                torch.cuda.synchronize()
+                dm = min(dm, (time.perf_counter() - t0) * 1e3)
+            same = again == streams
             qmpx = sum(p.shape[0] * p.shape[1] for p, _ in host) / 1e6
             dmpx = sum(host[k][0].shape[0] * host[k][0].shape[1] for k in dec_ids) / 1e6
-            # the batches give each page the bytes and pixels it gets alone
-            sample = dec_ids[:3]
-            same = all(streams[k] == j.encode_u8(host[k][0], host[k][1]) for k in range(min(3, len(host))))
-            same = same and all(np.array_equal(outs[k].cpu().numpy(), j.decode_u8(streams[k])) for k in sample)
-            small = min(range(len(host)), key=lambda k: host[k][0].size)
-            ok = streams[small] == ora.encode(host[small][0], host[small][1])
+            # every page: the queue's bytes and pixels equal the page alone
+            note("mixed queue: every page against its single call")
+            same_enc = all(streams[k] == j.encode_u8(host[k][0], host[k][1]) for k in range(len(host)))
+            same_dec = all(np.array_equal(outs[k].cpu().numpy(), j.decode_u8(streams[k])) for k in dec_ids)
+            # the oracle on the smallest gray and the smallest sRGB page
+            ocheck = {}
+            for kind in (2, 3):
+                ks = [k for k in range(len(host)) if host[k][0].ndim == kind]
+                if ks:
+                    k = min(ks, key=lambda i: host[i][0].size)
+                    ref = ora.encode(host[k][0], host[k][1])
+                    ok = streams[k] == ref
+                    if ok and k in dec_ids:
+                        ok = np.array_equal(outs[k].cpu().numpy(), ora.decode(ref))
+                    ocheck[f"page{k}_{'gray' if kind == 2 else 'srgb'}_q{host[k][1]}"] = bool(ok)
+            import ctypes
+
+            qc, qe = ctypes.c_uint32(), ctypes.c_int()
+            lib.jp2g_get_queue(ctypes.byref(qc), ctypes.byref(qe))
             configs["mixed_queue"] = {
-                "pages": len(host), "megapixels": round(qmpx, 1), "batch_pages": B,
+                "pages": len(host), "megapixels": round(qmpx, 1), "calls": "one encode_many, one decode_many",
+                "queue_chunk_blocks": int(qc.value), "queue_engines": int(qe.value),
                 "gray_pages": sum(1 for p, _ in host if p.ndim == 2),
                 "decoded_pages": len(dec_ids), "decode_rejected_as_by_reference": {str(k): v for k, v in rejected.items()},
                 "encode_s": round(em / 1e3, 3), "decode_s": round(dm / 1e3, 3),
                 "encode_mpx_s": round(qmpx / (em / 1e3), 1), "decode_mpx_s": round(dmpx / (dm / 1e3), 1),
-                "batched_equals_single": bool(same), "parity_vs_oracle_smallest_page": bool(ok)}
+                "repeat_identical": bool(same), "every_page_equals_single_call": bool(same_enc and same_dec),
+                "parity_vs_oracle": ocheck}
             del devp, outs
 
             # mixed serving: Poisson arrivals of 60 % encodes of the queue's

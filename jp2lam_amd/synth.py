@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["gradient_page", "document_page", "photo_page", "mixed_pages", "page_spec"]
+__all__ = ["gradient_page", "document_page", "photo_page", "mixed_pages", "mixed_pages_parallel", "page_spec"]
 
 
 def _clamp_u8(a: np.ndarray) -> np.ndarray:
@@ -134,10 +134,11 @@ def page_spec(index: int, seed: int = 4) -> dict:
     return {"width": w, "height": h, "kind": kind, "quality": q, "seed": seed * 1000 + index}
 
 
-def mixed_pages(count: int, seed: int = 4, scale: float = 1.0):
-    """Yield ``(pixels, quality)`` for the mixed-size page queue.  ``scale``
-    shrinks both dimensions (tests use small pages)."""
-    for i in range(count):
+def mixed_pages(count: int, seed: int = 4, scale: float = 1.0, start: int = 0):
+    """Yield ``(pixels, quality)`` for pages ``start .. start + count - 1``
+    of the mixed-size page queue.  ``scale`` shrinks both dimensions (tests
+    use small pages)."""
+    for i in range(start, start + count):
         s = page_spec(i, seed)
         w = max(2, int(s["width"] * scale) // 2 * 2)
         h = max(2, int(s["height"] * scale) // 2 * 2)
@@ -146,3 +147,21 @@ def mixed_pages(count: int, seed: int = 4, scale: float = 1.0):
         else:
             px = photo_page(w, h, s["seed"])
         yield px, s["quality"]
+
+
+def _mixed_page(args):
+    index, seed, scale = args
+    return next(mixed_pages(1, seed, scale, start=index))
+
+
+def mixed_pages_parallel(count: int, seed: int = 4, scale: float = 1.0, workers: int = 8):
+    """``list(mixed_pages(count, seed, scale))``, generated by ``workers``
+    processes (started fresh, so this is safe after the GPU is in use)."""
+    if workers <= 1 or count <= 1:
+        return list(mixed_pages(count, seed, scale))
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_mixed_page, [(i, seed, scale) for i in range(count)], chunksize=1))
+
