@@ -269,6 +269,20 @@ jp2lam_status jp2g_t1_encode_block(const int32_t* coeffs, uint32_t w, uint32_t h
                                    int32_t* info, int32_t* passes, uint8_t* bytes, size_t cap);
 jp2lam_status jp2g_t1_decode_block(uint32_t w, uint32_t h, uint32_t band, uint32_t mb, uint32_t zbp, uint32_t passes,
                                    const uint8_t* data, size_t len, int32_t* out);
+/* Tier-1 counters (src/encode/counters.rs:3-41, the crate's "counters"
+ * feature): while on, every encode adds {total blocks, empty blocks, MQ
+ * symbols, cleanup passes, significance passes, refinement passes, pass
+ * bytes} to process-wide totals, counted on the device (the MQ decisions of
+ * every pass formed again after coding; the coding kernels are unchanged).
+ * jp2g_counters copies the 7 totals, jp2g_reset_counters clears them
+ * (counters.rs:11-19). */
+void jp2g_set_counters(int on);
+void jp2g_counters(uint64_t* out7);
+void jp2g_reset_counters(void);
+/* MQ decisions of block `index` of the last encode (debug capture and
+ * counting on): all its passes, or only the passes PCRD kept (the ones the
+ * decoder reads back). */
+uint32_t jp2g_last_block_symbols(int index, int kept_only);
 /* Per-block record of the last encode while debug capture is on. */
 void jp2g_set_debug(int on);
 int jp2g_last_block_count(void);

@@ -135,6 +135,10 @@ SIGNATURES = [
     ("jp2g_set_hybrid_frac", None, [C.c_float]),
     ("jp2g_set_queue", None, [C.c_uint32, C.c_int]),
     ("jp2g_get_queue", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
+    ("jp2g_set_counters", None, [C.c_int]),
+    ("jp2g_counters", None, [C.c_void_p]),
+    ("jp2g_reset_counters", None, []),
+    ("jp2g_last_block_symbols", C.c_uint32, [C.c_int, C.c_int]),
     ("jp2g_last_block_count", C.c_int, []),
     ("jp2g_last_block_info", None, [C.c_int, C.c_void_p]),
     ("jp2g_last_block_passes", None, [C.c_int, C.c_void_p]),

@@ -788,6 +788,16 @@ void jp2g_set_hybrid_frac(float frac) {
     std::lock_guard<std::mutex> g(e.mu);
     e.hybrid_frac = frac > 0.0f && frac < 1.0f ? frac : 0.55f;
 }
+void jp2g_set_counters(int on) { set_t1_counting(on != 0); }
+void jp2g_counters(uint64_t* out7) {
+    if (out7) t1_counters(out7);
+}
+void jp2g_reset_counters(void) { reset_t1_counters(); }
+uint32_t jp2g_last_block_symbols(int i, int kept_only) {
+    const auto& v = engine().last_blocks();
+    if (i < 0 || (size_t)i >= v.size()) return 0;
+    return kept_only ? v[i].nsym_kept : v[i].nsym;
+}
 int jp2g_last_block_count(void) { return (int)engine().last_blocks().size(); }
 void jp2g_last_block_info(int i, int32_t* info) {
     const BlockStat& b = engine().last_blocks()[i];

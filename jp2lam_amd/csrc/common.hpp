@@ -45,6 +45,8 @@ void launch_t1_hybrid(const BlockJob* jobs, int n, const uint32_t* tabs, const u
                       const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, uint32_t* lists, float frac,
                       hipStream_t st);
 uint64_t t1_stream_bound(uint32_t mbp);
+// MQ decisions per pass of every block (counts: n * kMaxPasses words).
+void launch_t1_count(const BlockJob* jobs, int n, const uint32_t* tabs, uint32_t* counts, hipStream_t st);
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st);
 void launch_pcrd(int n, bool lossless, int ngroups, const BlockResult* res, const uint32_t* snaps,
                  const uint16_t* newsigs, const PcrdJob* jobs, HullPt* hull, uint8_t* hull_n, unsigned long long* gmax,

@@ -143,6 +143,27 @@ Engine::~Engine() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+namespace {
+std::atomic<bool> g_counting{false};
+std::mutex g_counter_mu;
+uint64_t g_counters[7] = {0, 0, 0, 0, 0, 0, 0};
+void add_t1_counters(const uint64_t* c) {
+    std::lock_guard<std::mutex> g(g_counter_mu);
+    for (int i = 0; i < 7; i++) g_counters[i] += c[i];
+}
+}  // namespace
+
+void set_t1_counting(bool on) { g_counting.store(on); }
+bool t1_counting() { return g_counting.load(); }
+void t1_counters(uint64_t* out7) {
+    std::lock_guard<std::mutex> g(g_counter_mu);
+    for (int i = 0; i < 7; i++) out7[i] = g_counters[i];
+}
+void reset_t1_counters() {
+    std::lock_guard<std::mutex> g(g_counter_mu);
+    for (int i = 0; i < 7; i++) g_counters[i] = 0;
+}
+
 Engine& engine() {
     static Engine* e = new Engine();
     return *e;
@@ -307,6 +328,8 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
     const size_t o_offs = L.add(4 * nb);
     const size_t o_packed = L.add(pack_total);
     const size_t o_sched = L.add(4 * t1_sched_words(nb));
+    const bool counting = t1_counting();
+    const size_t o_counts = counting ? L.add(sizeof(uint32_t) * kMaxPasses * nb) : 0;
     // Lane-per-block and hybrid Tier-1 (large batches) code formed symbol
     // streams, bounded per block by the band's Mb + 2 bit-planes; an
     // overflow falls back to the wave-per-block kernel below.  The coder is
@@ -597,6 +620,17 @@ t1_again:
                 (uint32_t*)(D + o_offs), (uint32_t*)(D + o_totals), (const uint8_t*)(D + o_slots), (uint8_t*)(D + o_packed),
                 st, false);
     J2L_HIP(hipEventRecord(ev.e[3], st));
+    // counters: MQ decisions per pass, formed again from the coefficients
+    // after the stages timed above (the coding kernels are untouched)
+    std::vector<uint32_t> pass_syms, pass_snaps;
+    if (counting) {
+        J2L_HIP(hipMemsetAsync(D + o_counts, 0, sizeof(uint32_t) * kMaxPasses * nb, st));
+        launch_t1_count((const BlockJob*)(D + o_jobs), nb, tables_, (uint32_t*)(D + o_counts), st);
+        pass_syms.resize((size_t)kMaxPasses * nb);
+        pass_snaps.resize((size_t)kMaxPasses * nb);
+        J2L_HIP(hipMemcpyAsync(pass_syms.data(), D + o_counts, 4 * pass_syms.size(), hipMemcpyDeviceToHost, st));
+        J2L_HIP(hipMemcpyAsync(pass_snaps.data(), D + o_snaps, 4 * pass_snaps.size(), hipMemcpyDeviceToHost, st));
+    }
 
     // ---- readback ----------------------------------------------------------------------
     if (o_res - o_sel == h_res - h_sel && o_totals - o_sel == h_tot - h_sel && o_lambda - o_sel == h_lam - h_sel) {
@@ -626,6 +660,34 @@ t1_again:
     const uint32_t* totals = (const uint32_t*)(P + h_tot);
     std::vector<Selection> selv(sel, sel + nb);
     std::vector<BlockResult> resv(res, res + nb);
+    if (counting) {
+        // t1.rs:270-336, :593-641: blocks, empty blocks, MQ decisions,
+        // passes by kind (cleanup first, then SP, MR, CL per plane) and pass
+        // bytes (the per-pass lengths of the single terminated segment)
+        uint64_t c[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int b = 0; b < nb; b++) {
+            const BlockResult& r = resv[b];
+            c[0]++;
+            if (r.mbp == 0) {
+                c[1]++;
+                continue;
+            }
+            const uint32_t* sy = pass_syms.data() + (size_t)b * kMaxPasses;
+            const uint32_t* sn = pass_snaps.data() + (size_t)b * kMaxPasses;
+            uint32_t prev = 0, sum = 0;
+            for (int k = 0; k < r.npasses; k++) {
+                c[2] += sy[k];
+                const int kind = k == 0 ? 2 : (k - 1) % 3;  // 0 SP, 1 MR, 2 CL
+                c[kind == 2 ? 3 : kind == 0 ? 4 : 5]++;
+                uint32_t len = sn[k] > prev ? sn[k] - prev : 0;
+                prev = sn[k];
+                if (k == r.npasses - 1) len = r.nbytes > sum ? r.nbytes - sum : 0;
+                sum += len;
+                c[6] += len;
+            }
+        }
+        add_t1_counters(c);
+    }
     std::vector<uint32_t> totv(totals, totals + imgs.size());
     last_lambdas_.assign((const double*)(P + h_lam), (const double*)(P + h_lam) + ngroups);
     std::vector<double> m_out(2 * m_seg.size());
@@ -711,6 +773,13 @@ t1_again:
                 bs.body_len = sl.body_len;
                 bs.max_mag = r.max_mag;
                 bs.nnz = r.nnz;
+                if (counting) {
+                    const uint32_t* sy = pass_syms.data() + (size_t)(im.blk_first + b) * kMaxPasses;
+                    for (int k = 0; k < r.npasses; k++) {
+                        bs.nsym += sy[k];
+                        if (k < sl.keep) bs.nsym_kept += sy[k];
+                    }
+                }
                 last_blocks_.push_back(bs);
             }
         }

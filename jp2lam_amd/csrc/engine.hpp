@@ -57,6 +57,7 @@ struct BlockStat {
     uint8_t comp, res, band, mbp, zbp, cls, npasses, keep;
     uint16_t x0, y0, x1, y1;
     uint32_t nbytes, body_len, max_mag, nnz;
+    uint32_t nsym, nsym_kept;  // MQ decisions, all passes / kept passes (counting on)
 };
 
 struct Arena {
@@ -123,6 +124,13 @@ private:
 };
 
 Engine& engine();
+// Tier-1 counters (encode/counters.rs:3-9), summed over every encode of
+// every engine while counting is on: {total blocks, empty blocks, MQ
+// symbols, cleanup passes, SP passes, MR passes, pass bytes}.
+void set_t1_counting(bool on);
+bool t1_counting();
+void t1_counters(uint64_t* out7);
+void reset_t1_counters();
 bool device_available(std::string* why = nullptr);
 
 }  // namespace j2l

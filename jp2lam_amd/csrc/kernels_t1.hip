@@ -1142,6 +1142,55 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     if (lane == 0) results[blk] = R;
 }
 
+// ---- Tier-1 counters (encode/counters.rs:3-9) ----------------------------------
+// MQ decisions per coding pass of every block: the same context formation as
+// the coders (every formed symbol is one MQ decision, mq/mod.rs:213-216),
+// with no coding.  Runs only when counting is on, after the coder, so the
+// coding kernels stay as they are.  counts[b * kMaxPasses + k].
+__global__ __launch_bounds__(64) void t1_count_kernel(const BlockJob* __restrict__ jobs,
+                                                      const uint32_t* __restrict__ tabs,
+                                                      uint32_t* __restrict__ counts) {
+    __shared__ uint32_t symw[kSymCap / 4];
+    uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
+    const int lane = lid();
+    const BlockJob J = jobs[blockIdx.x];
+    const int w = J.w, h = J.h;
+    uint32_t mag[64];
+    uint64_t neg;
+    BlockResult R;
+    const int mbp = load_block(J, mag, neg, R);
+    if (mbp == 0) return;
+    Tables t;
+    load_tables(t, tabs, J.band);
+    Col st = {0, 0, 0, 0};
+    uint32_t c0 = 0, c1 = 0, np = 0;
+    auto count = [&](uint32_t n) {
+        __syncthreads();
+        if (np < 64u) c0 = wlane(c0, n, np);
+        else c1 = wlane(c1, n, np - 64u);
+        np++;
+    };
+    const int top = mbp - 1;
+    for (int p = top; p >= 0; p--) {
+        uint64_t bits = 0;
+#pragma unroll
+        for (int y = 0; y < 64; y++) bits |= (uint64_t)((mag[y] >> p) & 1u) << y;
+        uint32_t ns;
+        if (p != top) {
+            count(form_spp(t, st, bits, neg, w, h, symb, ns));
+            __syncthreads();
+            count(form_mrp(st, bits, w, h, symb));
+            __syncthreads();
+        }
+        count(form_cup(t, st, bits, neg, w, h, symb, ns));
+        __syncthreads();
+        st.vis = 0;
+    }
+    uint32_t* cp = counts + (size_t)blockIdx.x * kMaxPasses;
+    if ((uint32_t)lane < np) cp[lane] = c0;
+    if ((uint32_t)lane + 64u < np && lane + 64 < kMaxPasses) cp[lane + 64] = c1;
+}
+
 // ---- batched Tier-1 encoder: context formation, then one lane per block ------
 //
 // For large batches the MQ coder runs one code-block per lane: a VALU
@@ -1920,6 +1969,9 @@ void launch_t1_hybrid(const BlockJob* jobs, int n, const uint32_t* tabs, const u
     if (n <= 0) return;
     t1_split_kernel<<<1, 1024, 0, st>>>(n, res, pend, frac, lists);
     t1_hybrid_kernel<<<n + (n + 63) / 64, 64, 0, st>>>(n, jobs, tabs, pool, soff16, pend, res, snaps, out, lists);
+}
+void launch_t1_count(const BlockJob* jobs, int n, const uint32_t* tabs, uint32_t* counts, hipStream_t st) {
+    if (n > 0) t1_count_kernel<<<n, 64, 0, st>>>(jobs, tabs, counts);
 }
 uint64_t t1_stream_bound(uint32_t mbp) { return stream_bound(mbp); }
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st) {

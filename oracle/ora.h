@@ -75,6 +75,7 @@ typedef struct {
     int body_len;              /* bytes emitted into the packet body */
     /* PCRD record of the last lossy encode (kept for the tests): the raw
      * curve, origin first (rate.rs:128-180), and the hull's pass counts. */
+    uint32_t nsym;             /* MQ symbols coded for the block, all passes */
     int raw_n, hull_n;
     uint32_t raw_bytes[ORA_MAX_PASSES + 1];
     double raw_dist[ORA_MAX_PASSES + 1];
@@ -131,6 +132,7 @@ typedef struct {
     uint8_t *out;
     size_t cap, pos; /* out[0] is the dummy pre-byte */
     uint8_t ctx[19];
+    uint64_t nsym;   /* symbols coded (counters.rs MQ_SYMBOLS, mq/mod.rs:213-216) */
 } ora_mqc;
 void ora_mqc_init(ora_mqc *m, uint8_t *buf, size_t cap);
 void ora_mqc_encode(ora_mqc *m, int ctx, int bit);

@@ -563,6 +563,27 @@ int ora_enc_block_rd(const ora_encoding *e, int i, uint32_t *raw_bytes, double *
     return b->raw_n;
 }
 double ora_enc_max_slope(const ora_encoding *e, int c) { return e->max_slope[c]; }
+/* encode/counters.rs:3-9 for this encode: total blocks, empty blocks, MQ
+ * symbols, cleanup / SP / MR passes (as coded, before truncation) and the
+ * coded bytes of every pass. */
+void ora_enc_counters(const ora_encoding *e, uint64_t *out) {
+    memset(out, 0, 7 * sizeof(uint64_t));
+    for (int i = 0; i < e->nblocks; i++) {
+        const ora_block *b = &e->blocks[i];
+        out[0]++;
+        if (b->mbp == 0) {
+            out[1]++;
+            continue;
+        }
+        out[2] += b->nsym;
+        for (int k = 0; k < b->full_npasses; k++) {
+            int kind = b->passes[k].kind;
+            out[kind == ORA_CL ? 3 : kind == ORA_SP ? 4 : 5]++;
+            out[6] += (uint64_t)b->full_lengths[k];
+        }
+    }
+}
+uint32_t ora_enc_block_nsym(const ora_encoding *e, int i) { return e->blocks[i].nsym; }
 void ora_enc_block_bytes(const ora_encoding *e, int i, uint8_t *out) {
     const ora_block *b = &e->blocks[i];
     if (b->nbytes) memcpy(out, b->bytes, (size_t)b->nbytes);
@@ -613,6 +634,7 @@ int ora_t1_encode_flat(const int32_t *coef, int w, int h, int band, int mb, int3
     info[2] = b.cls;
     info[3] = b.npasses;
     info[4] = b.nbytes;
+    info[5] = (int32_t)b.nsym;
     for (int k = 0; k < b.npasses; k++) {
         passes[5 * k + 0] = b.passes[k].kind;
         passes[5 * k + 1] = b.passes[k].bitplane;

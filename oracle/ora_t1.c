@@ -65,6 +65,7 @@ void ora_mqc_init(ora_mqc *m, uint8_t *buf, size_t cap) {
     m->cap = cap;
     m->pos = 0;
     m->out[0] = 0;
+    m->nsym = 0;
     reset_ctx(m->ctx);
 }
 
@@ -114,6 +115,7 @@ static void renorme(ora_mqc *m) {
 
 /* mq/mod.rs:213-242 */
 void ora_mqc_encode(ora_mqc *m, int ctx, int bit) {
+    m->nsym++;
     uint8_t s = m->ctx[ctx];
     uint32_t qe = st_qe(s);
     if (bit == st_mps(s)) {
@@ -539,6 +541,7 @@ int ora_t1_encode_block(const int32_t *coef, int w, int h, int band, int mb, ora
         clear_visited_all(&g);
     }
     uint8_t *seg;
+    blk->nsym = (uint32_t)mq.nsym;
     size_t total = ora_mqc_flush(&mq, &seg);
     /* t1.rs:586-600: lengths from pos snapshots, last pass absorbs the rest */
     size_t prev = 0, sum = 0, cum = 0;

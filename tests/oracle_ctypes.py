@@ -96,6 +96,8 @@ class Oracle:
                                            C.POINTER(C.c_uint16)]),
             "ora_enc_block_rd": (C.c_int, [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_int32)]),
             "ora_enc_max_slope": (C.c_double, [vp, C.c_int]),
+            "ora_enc_counters": (None, [vp, vp]),
+            "ora_enc_block_nsym": (C.c_uint32, [vp, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -224,7 +226,7 @@ class Oracle:
     def t1_encode(self, coeffs: np.ndarray, band: int, mb: int):
         c = np.ascontiguousarray(coeffs, dtype=np.int32)
         h, w = c.shape
-        info = np.zeros(5, dtype=np.int32)
+        info = np.zeros(6, dtype=np.int32)  # mbp zbp cls npasses nbytes mq_symbols
         passes = np.zeros(5 * 96, dtype=np.int32)
         cap = w * h * 4 + 1024
         out = np.zeros(cap, dtype=np.uint8)
@@ -350,3 +352,14 @@ class Encoding:
         hn = C.c_int32()
         n = self.o.L.ora_enc_block_rd(self.h, i, rb.ctypes.data, rdist.ctypes.data, hp.ctypes.data, C.byref(hn))
         return rb[:n], rdist[:n], hp[:hn.value]
+
+    def counters(self):
+        """encode/counters.rs totals of this encode: blocks, empty blocks, MQ
+        symbols, cleanup / SP / MR passes, pass bytes."""
+        a = np.zeros(7, np.uint64)
+        self.o.L.ora_enc_counters(self.h, a.ctypes.data)
+        return [int(v) for v in a]
+
+    def nsym(self, i) -> int:
+        return int(self.o.L.ora_enc_block_nsym(self.h, i))
+

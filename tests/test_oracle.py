@@ -437,3 +437,34 @@ def test_content_rgb_luma_and_colorfulness(ora):
     # luma bins 77, 149, 29, and 77 again for the clamped 4th pixel
     # (255, 0, 0): p = [0.5, 0.25, 0.25] -> 1.5 bits
     assert st[1] == 1.5 and st[0] == 0.0
+
+
+def test_mq_symbol_counts_follow_ebcot(ora):
+    """MQ decisions per block (encode/counters.rs MQ_SYMBOLS) on blocks whose
+    coding the EBCOT rules fix by hand (ISO 15444-1 D.3): [1] is one ZC and
+    one sign decision; [-3] adds one refinement on the next plane (its SP
+    and CL skip the significant sample); a 4x4 block with only (0,0) set is
+    a run-mode column (AGG, two UNI, the sign, ZC for rows 1-3), a column
+    beside it coded by ZC (4), and two run-mode columns with AGG 0."""
+    assert ora.t1_encode(np.array([[1]]), 0, 9)[0][5] == 2
+    assert ora.t1_encode(np.array([[-3]]), 0, 9)[0][5] == 3
+    a = np.zeros((4, 4), np.int32)
+    a[0, 0] = 1
+    assert ora.t1_encode(a, 0, 9)[0][5] == 7 + 4 + 1 + 1
+    # an empty block codes nothing
+    assert ora.t1_encode(np.zeros((4, 4), np.int32), 0, 9)[0][5] == 0
+
+
+def test_encode_counters_totals(ora):
+    """counters.rs totals over one encode: every block counted, cleanup
+    passes = sum of bit-planes, SP = MR = cleanup - non-empty blocks, pass
+    bytes = the blocks' segment bytes."""
+    px = synth.document_page(300, 200, seed=4)
+    _, enc = ora.encode(px, 85, keep=True)
+    tot, empty, syms, cl, sp, mr, nbytes = enc.counters()
+    infos = [enc.info(i) for i in range(enc.nblocks)]
+    assert tot == enc.nblocks and empty == sum(1 for v in infos if v[7] == 0)
+    assert cl == sum(int(v[7]) for v in infos)
+    assert sp == mr == cl - (tot - empty)
+    assert nbytes == sum(int(v[12]) for v in infos)
+    assert syms == sum(enc.nsym(i) for i in range(enc.nblocks)) > 0
