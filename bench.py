@@ -150,8 +150,13 @@ def main():
             "algorithmic_bytes": int(dbytes), "avg_ms": round(dms, 4),
             "note": "Tier-1 is bound by the serial MQ chain of its heaviest code-block, not by HBM"}
 
-    # Tier-1 work units: coefficient bit-planes (sum over blocks of w*h*mbp)
+    # Tier-1 work units, outside the timed region, from the device counters:
+    # coefficient bit-planes (sum over blocks of w*h*mbp) and MQ decisions,
+    # in all and on the critical path (the block with the most decisions:
+    # all passes for the encoder, the kept passes for the decoder)
     lib.jp2g_set_debug(1)
+    lib.jp2g_reset_counters()
+    lib.jp2g_set_counters(1)
     try:
         j.encode_many([((d_in.data_ptr(), H, W, 1), Q)], device=True)
         info = np.zeros(16, np.int32)
@@ -159,10 +164,33 @@ def main():
         for bi in range(lib.jp2g_last_block_count()):
             lib.jp2g_last_block_info(bi, info.ctypes.data)
             cbp += int(info[5] - info[3]) * int(info[6] - info[4]) * int(info[7])
+        nblk = lib.jp2g_last_block_count()
+        sym_all = [lib.jp2g_last_block_symbols(bi, 0) for bi in range(nblk)]
+        sym_kept = [lib.jp2g_last_block_symbols(bi, 1) for bi in range(nblk)]
+        ctr = np.zeros(7, np.uint64)
+        lib.jp2g_counters(ctr.ctypes.data)
     finally:
+        lib.jp2g_set_counters(0)
         lib.jp2g_set_debug(0)
     roof["t1_encode_coeff_bitplanes_per_s"] = round(cbp / (enc_t[1] / 1e3), 1) if enc_t[1] > 0 else None
     roof["t1_decode_coeff_bitplanes_per_s"] = round(cbp / (dec_t[1] / 1e3), 1) if dec_t[1] > 0 else None
+    crit_e, crit_d = max(sym_all), max(sym_kept)
+    roof["t1_critical_path"] = {
+        "note": "Tier-1 is a serial MQ chain per code-block: the kernel ends with its heaviest block, so its "
+                "time over that block's decisions is the cost per decision on the critical path",
+        "mq_decisions_per_page": int(ctr[2]), "mq_decisions_kept_per_page": int(sum(sym_kept)),
+        "encode_heaviest_block_decisions": int(crit_e),
+        "encode_ns_per_decision": round(enc_t[1] * 1e6 / crit_e, 1) if crit_e else None,
+        "decode_heaviest_block_decisions": int(crit_d),
+        "decode_ns_per_decision": round(dec_t[1] * 1e6 / crit_d, 1) if crit_d else None,
+        "encode_decisions_per_s": round(int(ctr[2]) / (enc_t[1] / 1e3), 1) if enc_t[1] > 0 else None,
+        "decode_decisions_per_s": round(sum(sym_kept) / (dec_t[1] / 1e3), 1) if dec_t[1] > 0 else None}
+    tfile_u = ROOT / "profiles" / "utilisation_t1_dwt_doc2550x3300_q85.json"
+    if tfile_u.exists() and (W, H, Q) == (2550, 3300, 85):
+        u = json.loads(tfile_u.read_text()).get(f"plain:{dom.split(' ')[0]}")
+        if u:
+            roof["utilisation_from_profile"] = {k: u[k] for k in ("valu_busy", "salu_busy", "waves_per_cu",
+                                                                  "lds_bank_conflict_pct") if k in u}
     stage_ms = {k: round(v[0], 4) for k, v in stages.items()}
     stage_ms["encode_host_tier2"] = round(enc_t[3], 4)
 
