@@ -188,6 +188,15 @@ double ora_quality_to_lambda(int quality, uint32_t pixels);
 double ora_calibrate_lambda(double max_slope, double raw_lambda, int quality, uint32_t pixels);
 int ora_pcrd_apply(ora_encoding *enc, const ora_plan *plan);
 
+/* timing-faithful dead work (ora_perceptual.c): the contrast-mask map and
+ * plan_coding_passes, run by ora_encode when faithful mode is on */
+void ora_set_faithful(int on);
+int ora_faithful(void);
+double ora_contrast_mask(const int32_t *const *planes, int ncomp, int colorspace, uint32_t w, uint32_t h);
+long ora_plan_coding_passes(const int32_t *coef, int n, int mbp);
+/* host threads of the OpenMP regions (0 = the runtime default) */
+void ora_set_threads(int n);
+
 /* codec */
 int ora_encode(const int32_t *const *planes, uint32_t w, uint32_t h, int ncomp, int colorspace,
                int quality, int format, ora_encoding *enc);

@@ -4,6 +4,7 @@
  * encode/backend/native/backend.rs:52-242, :436-517, layout.rs:37-173,
  * decode/mod.rs:56-129, decode/t1.rs:32-73 and decode/reconstruct.rs. */
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -76,6 +77,11 @@ static int subband_mb(const ora_plan *p, int c, int res, int band) {
 
 static int count_blocks(int x0, int x1, int cb) { return x1 > x0 ? (x1 - x0 + cb - 1) / cb : 0; }
 
+/* Results of the faithful-mode dead work land here, so it is not elided. */
+static volatile long faithful_sink;
+
+void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+
 int ora_encode(const int32_t *const *planes, uint32_t w, uint32_t h, int ncomp, int colorspace,
                int quality, int format, ora_encoding *enc) {
     memset(enc, 0, sizeof(*enc));
@@ -134,11 +140,15 @@ int ora_encode(const int32_t *const *planes, uint32_t w, uint32_t h, int ncomp, 
         for (int y = 0; y < bh; y++)
             memcpy(cb + y * bw, coef[blk->comp] + (size_t)(blk->y0 + y) * w + blk->x0, sizeof(int32_t) * (size_t)bw);
         ora_t1_encode_block(cb, bw, bh, blk->band, subband_mb(p, blk->comp, blk->res, blk->band), blk);
+        if (ora_faithful()) faithful_sink += ora_plan_coding_passes(cb, bw * bh, blk->mbp);
         free(cb);
     }
     for (int c = 0; c < ncomp; c++) free(coef[c]);
     free(coef);
     for (int c = 0; c < 3; c++) free(conv[c]);
+    /* backend.rs:650-652: the lossy path builds the contrast mask that
+     * rate.rs:60 then ignores (faithful mode only) */
+    if (ora_faithful() && !p->lossless) faithful_sink += (long)ora_contrast_mask(src, ncomp, p->domain, w, h);
     ora_pcrd_apply(enc, p);
     /* Tier-2: one packet per (resolution, component) (t2.rs:409-470) */
     obuf pay = {0};

@@ -97,6 +97,8 @@ class Oracle:
             "ora_enc_block_rd": (C.c_int, [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_int32)]),
             "ora_enc_max_slope": (C.c_double, [vp, C.c_int]),
             "ora_enc_counters": (None, [vp, vp]),
+            "ora_set_faithful": (None, [C.c_int]),
+            "ora_set_threads": (None, [C.c_int]),
             "ora_enc_block_nsym": (C.c_uint32, [vp, C.c_int]),
         }
         for name, (res, args) in sig.items():

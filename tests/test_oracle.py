@@ -485,3 +485,20 @@ def test_encode_counters_totals(ora):
     assert sp == mr == cl - (tot - empty)
     assert nbytes == sum(int(v[12]) for v in infos)
     assert syms == sum(enc.nsym(i) for i in range(enc.nblocks)) > 0
+
+
+@pytest.mark.parametrize("q", [85, 100])
+def test_faithful_mode_changes_no_byte(ora, q):
+    """The CPU baseline's faithful mode adds the reference's discarded work
+    (contrast-mask DCT, plan_coding_passes) and nothing else; one thread
+    gives the same bytes too."""
+    px = synth.photo_page(120, 90, seed=12)
+    lean = ora.encode(px, q)
+    try:
+        ora.L.ora_set_faithful(1)
+        assert ora.encode(px, q) == lean
+        ora.L.ora_set_threads(1)
+        assert ora.encode(px, q) == lean
+    finally:
+        ora.L.ora_set_faithful(0)
+        ora.L.ora_set_threads(0)
