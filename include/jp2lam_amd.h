@@ -234,7 +234,19 @@ typedef struct {
     double queue_ms, run_ms, total_ms;  /* submit->start, start->done, submit->done */
 } jp2lam_served;
 jp2lam_server* jp2lam_server_new(int slots, size_t mem_budget_mib, int max_batch);
-/* Return a request id, or 0 with jp2lam_last_error() set. */
+/* The same request loop on a simulated executor, for testing admission,
+ * cancellation and fault handling without a GPU: no device work; an
+ * encode's result is a 24-byte digest of the request (width, height,
+ * colour space, quality as u32, FNV-1a of the pixels), a decode's the
+ * stream's bytes as a len x 1 gray image, each batch after delay_us. */
+jp2lam_server* jp2lam_server_new_simulated(int slots, size_t mem_budget_mib, int max_batch, uint32_t delay_us);
+/* Return a request id, or 0 with jp2lam_last_error() set.  A request whose
+ * device-memory estimate exceeds one slot's share of the budget is refused
+ * here with INVALID_INPUT.  A device error while a batch runs fails that
+ * batch's requests with ENCODE_FAILED / DECODE_FAILED ("device error: ...")
+ * and the server goes on serving; JP2LAM_FAULT_INJECT="encode[:N]" or
+ * "decode[:N]", read when the server starts, injects such an error into the
+ * N-th batch call of that kind (default the first), for tests. */
 uint64_t jp2lam_server_submit_encode(jp2lam_server* srv, const uint8_t* pixels, uint32_t width, uint32_t height,
                                      uint32_t channels, uint8_t quality, uint8_t format, uint8_t on_device);
 uint64_t jp2lam_server_submit_decode(jp2lam_server* srv, const uint8_t* bytes, size_t len);
@@ -242,6 +254,13 @@ uint64_t jp2lam_server_submit_decode(jp2lam_server* srv, const uint8_t* bytes, s
 int jp2lam_server_wait(jp2lam_server* srv, uint64_t id, int timeout_ms, jp2lam_served* out);
 /* {submitted, completed, batches, largest batch} */
 void jp2lam_server_stats(jp2lam_server* srv, uint64_t* out4);
+/* Cancel a request that no slot has taken yet: 1 done (its wait then gives
+ * status IO, "request cancelled"), 0 already running or finished, -1 for
+ * an unknown id or null server. */
+int jp2lam_server_cancel(jp2lam_server* srv, uint64_t id);
+/* {requests refused at admission, cancelled, batch calls failed by a device
+ * error} */
+void jp2lam_server_faults(jp2lam_server* srv, uint64_t* out3);
 void jp2lam_server_free(jp2lam_server* srv);
 
 /* ---- device buffers (no counterpart in the reference) ----------------------

@@ -585,9 +585,16 @@ class Server:
     admitted as a slot frees, up to ``mem_budget_mib`` of device memory per
     server and ``max_batch`` requests per batch."""
 
-    def __init__(self, slots: int = 2, mem_budget_mib: int = 16384, max_batch: int = 16):
+    def __init__(self, slots: int = 2, mem_budget_mib: int = 16384, max_batch: int = 16,
+                 simulated_delay_us: Optional[int] = None):
+        """``simulated_delay_us``: run the request loop on the simulated
+        executor (no device work; for tests of admission, cancellation and
+        fault handling) with that delay per batch."""
         self._lib = _lib.load()
-        self._h = self._lib.jp2lam_server_new(slots, mem_budget_mib, max_batch)
+        if simulated_delay_us is None:
+            self._h = self._lib.jp2lam_server_new(slots, mem_budget_mib, max_batch)
+        else:
+            self._h = self._lib.jp2lam_server_new_simulated(slots, mem_budget_mib, max_batch, int(simulated_delay_us))
         self._kinds = {}
         if not self._h:
             raise IoError(self._lib.jp2lam_last_error().decode())
@@ -634,6 +641,18 @@ class Server:
             value = arr.reshape(out.height, out.width) if out.channels == 1 else \
                 arr.reshape(out.height, out.width, out.channels)
         return Served(value, out.queue_ms, out.run_ms, out.total_ms, out.batch_size)
+
+    def cancel(self, rid: int) -> bool:
+        """True if the request was still queued and is now cancelled."""
+        rc = self._lib.jp2lam_server_cancel(self._h, rid)
+        if rc < 0:
+            raise InvalidInput(self._lib.jp2lam_last_error().decode())
+        return rc == 1
+
+    def faults(self) -> dict:
+        v = (C.c_uint64 * 3)()
+        self._lib.jp2lam_server_faults(self._h, v)
+        return {"rejected": int(v[0]), "cancelled": int(v[1]), "device_faults": int(v[2])}
 
     def stats(self) -> dict:
         v = (C.c_uint64 * 4)()

@@ -127,6 +127,9 @@ SIGNATURES = [
     ("jp2lam_server_submit_decode", C.c_uint64, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("jp2lam_server_wait", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(ServedC)]),
     ("jp2lam_server_stats", None, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("jp2lam_server_new_simulated", C.c_void_p, [C.c_int, C.c_size_t, C.c_int, C.c_uint32]),
+    ("jp2lam_server_cancel", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("jp2lam_server_faults", None, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("jp2lam_server_free", None, [C.c_void_p]),
     ("jp2g_set_debug", None, [C.c_int]),
     ("jp2g_set_t1_mode", None, [C.c_int, C.c_uint32]),

@@ -887,6 +887,23 @@ jp2lam_server* jp2lam_server_new(int slots, size_t mem_budget_mib, int max_batch
     });
     return out;
 }
+jp2lam_server* jp2lam_server_new_simulated(int slots, size_t mem_budget_mib, int max_batch, uint32_t delay_us) {
+    jp2lam_server* out = nullptr;
+    guarded([&] {
+        out = new jp2lam_server{std::make_unique<Server>(slots, mem_budget_mib << 20, max_batch, true, delay_us)};
+        return JP2LAM_OK;
+    });
+    return out;
+}
+int jp2lam_server_cancel(jp2lam_server* srv, uint64_t id) {
+    int rc = -1;
+    guarded([&] {
+        if (!srv) return fail(INVALID_INPUT, "null argument");
+        rc = srv->s->cancel(id);
+        return JP2LAM_OK;
+    });
+    return rc;
+}
 uint64_t jp2lam_server_submit_encode(jp2lam_server* srv, const uint8_t* pixels, uint32_t width, uint32_t height,
                                      uint32_t channels, uint8_t quality, uint8_t format, uint8_t on_device) {
     uint64_t id = 0;
@@ -956,6 +973,13 @@ void jp2lam_server_stats(jp2lam_server* srv, uint64_t* out4) {
     out4[1] = st.completed;
     out4[2] = st.batches;
     out4[3] = st.max_batch;
+}
+void jp2lam_server_faults(jp2lam_server* srv, uint64_t* out3) {
+    if (!srv || !out3) return;
+    ServerStats st = srv->s->stats();
+    out3[0] = st.rejected;
+    out3[1] = st.cancelled;
+    out3[2] = st.device_faults;
 }
 void jp2lam_server_free(jp2lam_server* srv) { delete srv; }
 

@@ -56,14 +56,38 @@ struct Request {
     DecOut dout;
 };
 
+// What a serving slot runs its batches on: a device engine, or, for tests
+// of the request loop on a machine without a GPU, a simulated executor
+// (no device work: an encode's result is a 24-byte digest of the request,
+// a decode's the stream's bytes as a 1-row gray image, after a fixed delay).
+class Runner {
+public:
+    virtual ~Runner() = default;
+    virtual std::vector<EncOut> encode(const std::vector<EncItem>& items) = 0;
+    virtual std::vector<DecOut> decode(const std::vector<DecItem>& items) = 0;
+};
+std::unique_ptr<Runner> make_engine_runner();
+std::unique_ptr<Runner> make_simulated_runner(uint32_t delay_us);
+
+// Injected device faults (JP2LAM_FAULT_INJECT="encode" | "decode" |
+// "encode:N" | "decode:N", read when the server starts): the N-th (default
+// first) batch call of that kind fails as a device error would.
+struct FaultPlan {
+    int encode_at = 0, decode_at = 0;  // 1-based call number, 0 = never
+    static FaultPlan from_env();
+};
+
 struct ServerStats {
     uint64_t submitted = 0, completed = 0, batches = 0;
     uint64_t max_batch = 0;
+    uint64_t rejected = 0, cancelled = 0, device_faults = 0;
 };
 
 class Server {
 public:
-    Server(int slots, size_t mem_budget_bytes, int max_batch);
+    // simulated: slots run the simulated executor with `sim_delay_us` per
+    // batch instead of device engines (tests of the request loop only).
+    Server(int slots, size_t mem_budget_bytes, int max_batch, bool simulated = false, uint32_t sim_delay_us = 0);
     ~Server();
     // Encode request: host pixels (u8, interleaved, channels 1 or 3) are
     // copied; device pixels (on_device) must stay valid until completion.
@@ -73,6 +97,9 @@ public:
     // Waits up to timeout_ms (< 0: no limit) and moves the result out.
     // Returns nullptr on timeout; throws Error for an unknown id.
     std::shared_ptr<Request> wait(uint64_t id, int timeout_ms);
+    // 1: removed before it started (it completes with status IO, "request
+    // cancelled"); 0: already running or done; throws for an unknown id.
+    int cancel(uint64_t id);
     ServerStats stats();
 
 private:
@@ -83,7 +110,10 @@ private:
     std::condition_variable cv_work_, cv_done_;
     std::deque<std::shared_ptr<Request>> queue_;
     std::unordered_map<uint64_t, std::shared_ptr<Request>> live_;
-    std::vector<std::unique_ptr<Engine>> engines_;
+    std::vector<std::unique_ptr<Runner>> runners_;
+    FaultPlan faults_;
+    bool simulated_ = false;
+    int enc_calls_ = 0, dec_calls_ = 0;
     std::vector<std::thread> threads_;
     bool stop_ = false;
     uint64_t next_id_ = 1;
