@@ -41,6 +41,13 @@ static void yuv_to_rgb(const int32_t *const *in, size_t n, int32_t **out) {
     }
 }
 
+/* yuv_to_rgb on flat planes (3 x n), for the context.rs:107-157 test */
+void ora_ycbcr_to_rgb(const int32_t *planes_flat, size_t n, int32_t *out_flat) {
+    const int32_t *in[3] = {planes_flat, planes_flat + n, planes_flat + 2 * n};
+    int32_t *out[3] = {out_flat, out_flat + n, out_flat + 2 * n};
+    yuv_to_rgb(in, n, out);
+}
+
 /* Quantised Mallat coefficient plane of one component
  * (backend.rs:52-84 lossy, :118-159 lossless). */
 static void component_coefficients(const ora_plan *p, const int32_t *const *src, int c, int32_t *out) {

@@ -98,6 +98,7 @@ class Oracle:
             "ora_enc_max_slope": (C.c_double, [vp, C.c_int]),
             "ora_enc_counters": (None, [vp, vp]),
             "ora_set_faithful": (None, [C.c_int]),
+            "ora_ycbcr_to_rgb": (None, [vp, C.c_size_t, vp]),
             "ora_set_threads": (None, [C.c_int]),
             "ora_enc_block_nsym": (C.c_uint32, [vp, C.c_int]),
         }

@@ -502,3 +502,16 @@ def test_faithful_mode_changes_no_byte(ora, q):
     finally:
         ora.L.ora_set_faithful(0)
         ora.L.ora_set_threads(0)
+
+
+def test_ycbcr_to_rgb_reference_case(ora):
+    """context.rs:107-157: Y (64, 200), Cb (128, 90), Cr (128, 170) become
+    R (64, 255), G (64, 184), B (64, 132) before the encode."""
+    planes = np.array([[64, 200], [128, 90], [128, 170]], np.int32)
+    out = np.zeros_like(planes)
+    ora.L.ora_ycbcr_to_rgb(planes.ctypes.data, 2, out.ctypes.data)
+    assert out.tolist() == [[64, 255], [64, 184], [64, 132]]
+    # neutral chroma leaves gray; the integer BT.601 constants at the rails
+    grey = np.array([[0, 128, 255], [128] * 3, [128] * 3], np.int32)
+    ora.L.ora_ycbcr_to_rgb(grey.ctypes.data, 3, (o := np.zeros_like(grey)).ctypes.data)
+    assert o.tolist() == [[0, 128, 255]] * 3
