@@ -322,6 +322,17 @@ double jp2g_last_lambda(int group);
  * each, the rest by lanes, in one launch.  Output is the same in every
  * mode. */
 void jp2g_set_t1_mode(int mode, uint32_t lane_min_blocks);
+/* No counterpart in the reference (a CPU library).  Each engine runs its
+ * device launch sequence (colour transform, DWT levels, Tier-1, PCRD and
+ * packing for an encode; Tier-1, inverse DWT and pixel output for a decode)
+ * as a HIP graph once the same sequence (same arena offsets, counts and
+ * grids, i.e. a batch of the same shape) has been seen twice; other batches
+ * run eagerly.  Output is the same either way.  Off by default (measured no
+ * faster than eager launches); jp2g_set_graphs(1) turns the graphs on for
+ * every engine, jp2g_set_graphs(0) off; jp2g_graph_stats reports how many
+ * sequences were captured and how many graph launches ran. */
+void jp2g_set_graphs(int on);
+void jp2g_graph_stats(uint64_t* captures, uint64_t* launches);
 /* 1 when the last encode ran the lane-per-block coder. */
 int jp2g_last_t1_lanes(void);
 /* Tier-1 coder of the last encode: 1 wave per block, 2 lane per block,

@@ -138,6 +138,8 @@ SIGNATURES = [
     ("jp2g_set_hybrid_frac", None, [C.c_float]),
     ("jp2g_set_queue", None, [C.c_uint32, C.c_int]),
     ("jp2g_get_queue", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
+    ("jp2g_set_graphs", None, [C.c_int]),
+    ("jp2g_graph_stats", None, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("jp2g_set_counters", None, [C.c_int]),
     ("jp2g_counters", None, [C.c_void_p]),
     ("jp2g_reset_counters", None, []),

@@ -788,6 +788,8 @@ void jp2g_set_hybrid_frac(float frac) {
     std::lock_guard<std::mutex> g(e.mu);
     e.hybrid_frac = frac > 0.0f && frac < 1.0f ? frac : 0.55f;
 }
+void jp2g_set_graphs(int on) { set_graphs(on != 0); }
+void jp2g_graph_stats(uint64_t* captures, uint64_t* launches) { graph_stats(captures, launches); }
 void jp2g_set_counters(int on) { set_t1_counting(on != 0); }
 void jp2g_counters(uint64_t* out7) {
     if (out7) t1_counters(out7);

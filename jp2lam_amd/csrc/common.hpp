@@ -45,6 +45,8 @@ void launch_t1_hybrid(const BlockJob* jobs, int n, const uint32_t* tabs, const u
                       const uint32_t* pend, BlockResult* res, uint32_t* snaps, uint8_t* out, uint32_t* lists, float frac,
                       hipStream_t st);
 uint64_t t1_stream_bound(uint32_t mbp);
+// Zero n 64-bit words on the stream (a kernel, so it can sit in a graph).
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 // MQ decisions per pass of every block (counts: n * kMaxPasses words).
 void launch_t1_count(const BlockJob* jobs, int n, const uint32_t* tabs, uint32_t* counts, hipStream_t st);
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st);

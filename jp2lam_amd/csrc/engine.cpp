@@ -50,20 +50,6 @@ void copy_staged(const std::vector<Staged>& staged, char* D, char* P, hipStream_
     }
 }
 
-struct Events {
-    hipEvent_t e[8];
-    Events() {
-        for (auto& x : e) J2L_HIP(hipEventCreate(&x));
-    }
-    ~Events() {
-        for (auto& x : e) (void)hipEventDestroy(x);
-    }
-    float ms(int a, int b) {
-        float t = 0;
-        J2L_HIP(hipEventElapsedTime(&t, e[a], e[b]));
-        return t;
-    }
-};
 
 // context.rs:39-70: integer BT.601 YCbCr -> RGB for the hidden colour spaces
 void yuv_to_rgb(const int32_t* src, size_t n, std::vector<int32_t>& out) {
@@ -138,7 +124,68 @@ Engine::T1Choice Engine::tuned_t1_choice(int nblocks) {
     return c;
 }
 
+// ---- launch sequences as HIP graphs -------------------------------------------------
+// Off by default: on the MI355X the graphs measured no faster than the
+// eager launches (a 64x64 page through a 2550x3300 one, within 2 %), and the
+// capture of each new shape added latency under mixed-shape serving.
+static std::atomic<bool> g_graphs{false};
+static std::atomic<uint64_t> g_captures{0}, g_replays{0};
+void set_graphs(bool on) { g_graphs.store(on); }
+void graph_stats(uint64_t* captures, uint64_t* replays) {
+    if (captures) *captures = g_captures.load();
+    if (replays) *replays = g_replays.load();
+}
+
+GraphCache::~GraphCache() { clear(); }
+
+void GraphCache::clear() {
+    for (auto& kv : exec_) (void)hipGraphExecDestroy(kv.second);
+    exec_.clear();
+    seen_.clear();
+}
+
+void GraphCache::run(hipStream_t st, const std::function<void(bool, std::vector<uint64_t>&)>& seq) {
+    std::vector<uint64_t> sig;
+    seq(false, sig);
+    if (g_graphs.load()) {
+        auto it = exec_.find(sig);
+        if (it != exec_.end()) {
+            J2L_HIP(hipGraphLaunch(it->second, st));
+            g_replays++;
+            return;
+        }
+        // capture a sequence on its second occurrence; one-off batch shapes
+        // (and the first call of each) run eagerly
+        if (++seen_[sig] >= 2 && exec_.size() < kMaxGraphs) {
+            J2L_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            std::vector<uint64_t> again;
+            hipGraph_t g = nullptr;
+            try {
+                seq(true, again);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            J2L_HIP(hipStreamEndCapture(st, &g));
+            hipGraphExec_t x = nullptr;
+            const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            J2L_HIP(e);
+            exec_.emplace(std::move(sig), x);
+            g_captures++;
+            g_replays++;
+            J2L_HIP(hipGraphLaunch(x, st));
+            return;
+        }
+        if (seen_.size() > 4 * kMaxGraphs) seen_.clear();
+    }
+    std::vector<uint64_t> unused;
+    seq(true, unused);
+}
+
 Engine::~Engine() {
+    graphs_.clear();
     if (tables_) (void)hipFree(tables_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -551,108 +598,159 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         if (im.in_bytes) up(im.in_off, im.src, im.in_bytes);
 
     // ---- device pipeline -------------------------------------------------------------
-    Events ev;
-    J2L_HIP(hipEventRecord(ev.e[0], st));
-    for (int kind = 0; kind < 2; kind++) {
-        if (pseg_k[kind].empty()) continue;
-        launch_prepare((const PixSeg*)(D + o_pseg_k[kind]), (int)pseg_k[kind].size(), pblk_k[kind], st);
-        for (int k = 0; k < maxL; k++)
-            if (!fseg_k[kind][k].empty())
-                launch_fwd_level(kind == 1, (const DwtSeg*)(D + o_fseg_k[kind][k]), (int)fseg_k[kind][k].size(),
-                                 ftile_k[kind][k], st);
-    }
-    // images without any decomposition level: the coefficient plane is the
-    // prepared plane itself
-    for (auto& im : imgs)
-        if (im.plan.levels == 0)
-            for (int c = 0; c < im.plan.ncomp; c++)
-                J2L_HIP(hipMemcpyAsync(D + im.p_off[c], D + im.x_off[c], im.n * 4, hipMemcpyDeviceToDevice, st));
-    const unsigned long long zero64 = 0;
-    // formed-stream coders: 2 = lane per block, 4 = hybrid
-    auto run_streams = [&](int which, float split) {
-        J2L_HIP(hipMemcpyAsync(D + o_pused, &zero64, 8, hipMemcpyHostToDevice, st));
-        launch_t1_form((const BlockJob*)(D + o_jobs), nb, tables_, (BlockResult*)(D + o_res), (uint32_t*)(D + o_pend),
-                       (uint16_t*)(D + o_ns), (uint32_t*)(D + o_soff), (uint8_t*)(D + o_pool),
-                       (unsigned long long*)(D + o_pused), pool_cap, st);
-        if (which == 2)
-            launch_t1_mq((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
-                         (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
-                         (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), st);
-        else
-            launch_t1_hybrid((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
-                             (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
-                             (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), (uint32_t*)(D + o_lists), split, st);
+    // The launch sequence is written once and run either eagerly or through
+    // the engine's graph cache: each step appends what identifies it (kind,
+    // arena offsets, counts, grid sizes) to `sig`, and issues it when `go`.
+    Events& ev = ev_;
+    auto dwt_seq = [&](bool go, std::vector<uint64_t>& sig) {
+        sig.insert(sig.end(), {1, (uint64_t)(uintptr_t)D, (uint64_t)(uintptr_t)tables_});
+        for (int kind = 0; kind < 2; kind++) {
+            if (pseg_k[kind].empty()) continue;
+            sig.insert(sig.end(), {2, (uint64_t)kind, o_pseg_k[kind], pseg_k[kind].size(), pblk_k[kind]});
+            if (go) launch_prepare((const PixSeg*)(D + o_pseg_k[kind]), (int)pseg_k[kind].size(), pblk_k[kind], st);
+            for (int k = 0; k < maxL; k++)
+                if (!fseg_k[kind][k].empty()) {
+                    sig.insert(sig.end(), {3, (uint64_t)kind, o_fseg_k[kind][k], fseg_k[kind][k].size(), ftile_k[kind][k]});
+                    if (go)
+                        launch_fwd_level(kind == 1, (const DwtSeg*)(D + o_fseg_k[kind][k]), (int)fseg_k[kind][k].size(),
+                                         ftile_k[kind][k], st);
+                }
+        }
+        // images without any decomposition level: the coefficient plane is the
+        // prepared plane itself
+        for (auto& im : imgs)
+            if (im.plan.levels == 0)
+                for (int c = 0; c < im.plan.ncomp; c++) {
+                    sig.insert(sig.end(), {4, im.p_off[c], im.x_off[c], im.n});
+                    if (go) J2L_HIP(hipMemcpyAsync(D + im.p_off[c], D + im.x_off[c], im.n * 4, hipMemcpyDeviceToDevice, st));
+                }
     };
-    int run = coder;
-t1_again:
-    J2L_HIP(hipEventRecord(ev.e[1], st));
-    if (run != 1) {
-        run_streams(run, frac);
-    } else {
-        launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots), (BlockResult*)(D + o_res),
-                         (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns), (uint32_t*)(D + o_sched), st);
-    }
-    J2L_HIP(hipEventRecord(ev.e[2], st));
-    // PCRD: lossless images keep every pass; lossy ones go through the hull.
     bool any_lossy = false, any_lossless = false;
     for (auto& im : imgs) (im.plan.lossless ? any_lossless : any_lossy) = true;
-    if (any_lossy)
-        launch_pcrd(nb, false, (int)ngroups, (const BlockResult*)(D + o_res), (const uint32_t*)(D + o_snaps),
-                    (const uint16_t*)(D + o_ns), (const PcrdJob*)(D + o_pjobs), (HullPt*)(D + o_hull),
-                    (uint8_t*)(D + o_hulln), (unsigned long long*)(D + o_gmax), (const double*)(D + o_rawf),
-                    (double*)(D + o_lambda), (Selection*)(D + o_sel), st);
-    if (any_lossless) {
-        // lossless blocks: select "all passes" for the lossless images only
-        for (auto& im : imgs) {
-            if (!im.plan.lossless) continue;
-            launch_pcrd((int)im.blk_count, true, 0, (const BlockResult*)(D + o_res) + im.blk_first,
-                        (const uint32_t*)(D + o_snaps) + (size_t)im.blk_first * kMaxPasses, nullptr,
-                        (const PcrdJob*)(D + o_pjobs) + im.blk_first, nullptr, nullptr, nullptr, nullptr, nullptr,
-                        (Selection*)(D + o_sel) + im.blk_first, st);
+    auto t1_seq = [&](bool go, std::vector<uint64_t>& sig, int which) {
+        sig.insert(sig.end(), {5, (uint64_t)which, (uint64_t)nb, pool_cap, (uint64_t)(frac * 1e6f)});
+        if (which != 1) {
+            // formed-stream coders: 2 = lane per block, 4 = hybrid; the pool
+            // counter is cleared by a kernel (no host copy or memset node)
+            if (go) {
+                launch_zero_u64((unsigned long long*)(D + o_pused), 1, st);
+                launch_t1_form((const BlockJob*)(D + o_jobs), nb, tables_, (BlockResult*)(D + o_res),
+                               (uint32_t*)(D + o_pend), (uint16_t*)(D + o_ns), (uint32_t*)(D + o_soff),
+                               (uint8_t*)(D + o_pool), (unsigned long long*)(D + o_pused), pool_cap, st);
+                if (which == 2)
+                    launch_t1_mq((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
+                                 (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend), (BlockResult*)(D + o_res),
+                                 (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots), st);
+                else
+                    launch_t1_hybrid((const BlockJob*)(D + o_jobs), nb, tables_, (const uint8_t*)(D + o_pool),
+                                     (const uint32_t*)(D + o_soff), (const uint32_t*)(D + o_pend),
+                                     (BlockResult*)(D + o_res), (uint32_t*)(D + o_snaps), (uint8_t*)(D + o_slots),
+                                     (uint32_t*)(D + o_lists), frac, st);
+            }
+        } else if (go) {
+            launch_t1_encode((const BlockJob*)(D + o_jobs), nb, tables_, (uint8_t*)(D + o_slots),
+                             (BlockResult*)(D + o_res), (uint32_t*)(D + o_snaps), (uint16_t*)(D + o_ns),
+                             (uint32_t*)(D + o_sched), st);
         }
-    }
-    launch_pack(nb, (int)imgs.size(), (const uint32_t*)(D + o_imgfirst), (const uint32_t*)(D + o_blkimg),
-                (const uint64_t*)(D + o_packbase), (const BlockJob*)(D + o_jobs), (const Selection*)(D + o_sel),
-                (uint32_t*)(D + o_offs), (uint32_t*)(D + o_totals), (const uint8_t*)(D + o_slots), (uint8_t*)(D + o_packed),
-                st, true);
-    launch_pack(nb, (int)imgs.size(), (const uint32_t*)(D + o_imgfirst), (const uint32_t*)(D + o_blkimg),
-                (const uint64_t*)(D + o_packbase), (const BlockJob*)(D + o_jobs), (const Selection*)(D + o_sel),
-                (uint32_t*)(D + o_offs), (uint32_t*)(D + o_totals), (const uint8_t*)(D + o_slots), (uint8_t*)(D + o_packed),
-                st, false);
-    J2L_HIP(hipEventRecord(ev.e[3], st));
-    // counters: MQ decisions per pass, formed again from the coefficients
-    // after the stages timed above (the coding kernels are untouched)
+    };
+    auto pcrd_seq = [&](bool go, std::vector<uint64_t>& sig) {
+        // PCRD: lossless images keep every pass; lossy ones go through the hull.
+        sig.insert(sig.end(), {6, (uint64_t)any_lossy, (uint64_t)any_lossless, ngroups, imgs.size()});
+        if (any_lossy && go)
+            launch_pcrd(nb, false, (int)ngroups, (const BlockResult*)(D + o_res), (const uint32_t*)(D + o_snaps),
+                        (const uint16_t*)(D + o_ns), (const PcrdJob*)(D + o_pjobs), (HullPt*)(D + o_hull),
+                        (uint8_t*)(D + o_hulln), (unsigned long long*)(D + o_gmax), (const double*)(D + o_rawf),
+                        (double*)(D + o_lambda), (Selection*)(D + o_sel), st);
+        if (any_lossless) {
+            // lossless blocks: select "all passes" for the lossless images only
+            for (auto& im : imgs) {
+                if (!im.plan.lossless) continue;
+                sig.insert(sig.end(), {7, im.blk_first, im.blk_count});
+                if (go)
+                    launch_pcrd((int)im.blk_count, true, 0, (const BlockResult*)(D + o_res) + im.blk_first,
+                                (const uint32_t*)(D + o_snaps) + (size_t)im.blk_first * kMaxPasses, nullptr,
+                                (const PcrdJob*)(D + o_pjobs) + im.blk_first, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, (Selection*)(D + o_sel) + im.blk_first, st);
+            }
+        }
+        if (go)
+            for (int pass = 0; pass < 2; pass++)
+                launch_pack(nb, (int)imgs.size(), (const uint32_t*)(D + o_imgfirst), (const uint32_t*)(D + o_blkimg),
+                            (const uint64_t*)(D + o_packbase), (const BlockJob*)(D + o_jobs),
+                            (const Selection*)(D + o_sel), (uint32_t*)(D + o_offs), (uint32_t*)(D + o_totals),
+                            (const uint8_t*)(D + o_slots), (uint8_t*)(D + o_packed), st, pass == 0);
+    };
+    // every arena offset the steps use, once: the signature then names the
+    // whole launch sequence exactly
+    const std::vector<uint64_t> layout_sig = {
+        L.used, o_jobs, o_pjobs, o_gmax, o_rawf, o_imgfirst, o_blkimg, o_packbase, o_sel, o_res, o_totals, o_lambda,
+        o_snaps, o_ns, o_slots, o_hull, o_hulln, o_offs, o_packed, o_sched, o_pend, o_soff, o_pused, o_pool, o_lists};
+    // Each stage is one graph (or its launches), with the stage events
+    // recorded between them on the stream.
+    auto staged_run = [&](const std::function<void(bool, std::vector<uint64_t>&)>& seq, bool graph) {
+        if (!graph) {
+            std::vector<uint64_t> unused;
+            seq(true, unused);
+            return;
+        }
+        graphs_.run(st, [&](bool go, std::vector<uint64_t>& sig) {
+            sig.insert(sig.end(), layout_sig.begin(), layout_sig.end());
+            seq(go, sig);
+        });
+    };
+    int run = coder;
+    auto coding = [&](int which, bool graph) {
+        J2L_HIP(hipEventRecord(ev.e[1], st));
+        staged_run([&](bool go, std::vector<uint64_t>& sig) { t1_seq(go, sig, which); }, graph);
+        J2L_HIP(hipEventRecord(ev.e[2], st));
+        staged_run(pcrd_seq, graph);
+        J2L_HIP(hipEventRecord(ev.e[3], st));
+    };
+    J2L_HIP(hipEventRecord(ev.e[0], st));
+    staged_run(dwt_seq, true);
+    coding(run, true);
     std::vector<uint32_t> pass_syms, pass_snaps;
-    if (counting) {
-        J2L_HIP(hipMemsetAsync(D + o_counts, 0, sizeof(uint32_t) * kMaxPasses * nb, st));
-        launch_t1_count((const BlockJob*)(D + o_jobs), nb, tables_, (uint32_t*)(D + o_counts), st);
-        pass_syms.resize((size_t)kMaxPasses * nb);
-        pass_snaps.resize((size_t)kMaxPasses * nb);
-        J2L_HIP(hipMemcpyAsync(pass_syms.data(), D + o_counts, 4 * pass_syms.size(), hipMemcpyDeviceToHost, st));
-        J2L_HIP(hipMemcpyAsync(pass_snaps.data(), D + o_snaps, 4 * pass_snaps.size(), hipMemcpyDeviceToHost, st));
-    }
-
-    // ---- readback ----------------------------------------------------------------------
-    if (o_res - o_sel == h_res - h_sel && o_totals - o_sel == h_tot - h_sel && o_lambda - o_sel == h_lam - h_sel) {
-        J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, o_lambda + 8 * ngroups - o_sel, hipMemcpyDeviceToHost, st));
-    } else {
-        J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, sizeof(Selection) * nb, hipMemcpyDeviceToHost, st));
-        J2L_HIP(hipMemcpyAsync(P + h_res, D + o_res, sizeof(BlockResult) * nb, hipMemcpyDeviceToHost, st));
-        J2L_HIP(hipMemcpyAsync(P + h_tot, D + o_totals, 4 * imgs.size(), hipMemcpyDeviceToHost, st));
-        J2L_HIP(hipMemcpyAsync(P + h_lam, D + o_lambda, 8 * ngroups, hipMemcpyDeviceToHost, st));
-    }
-    J2L_HIP(hipStreamSynchronize(st));
-    t_dwt_ms = ev.ms(0, 1);
-    t_t1_ms = ev.ms(1, 2);
-    t_pcrd_ms = ev.ms(2, 3);
-    const Selection* sel = (const Selection*)(P + h_sel);
-    const BlockResult* res = (const BlockResult*)(P + h_res);
+    const Selection* sel = nullptr;
+    const BlockResult* res = nullptr;
+    // after the pipeline: counters (their own launches, outside the
+    // sequence), readback, and the stage times from the engine's events
+    auto finish = [&] {
+        // counters: MQ decisions per pass, formed again from the coefficients
+        // after the stages timed above (the coding kernels are untouched)
+        if (counting) {
+            J2L_HIP(hipMemsetAsync(D + o_counts, 0, sizeof(uint32_t) * kMaxPasses * nb, st));
+            launch_t1_count((const BlockJob*)(D + o_jobs), nb, tables_, (uint32_t*)(D + o_counts), st);
+            pass_syms.resize((size_t)kMaxPasses * nb);
+            pass_snaps.resize((size_t)kMaxPasses * nb);
+            J2L_HIP(hipMemcpyAsync(pass_syms.data(), D + o_counts, 4 * pass_syms.size(), hipMemcpyDeviceToHost, st));
+            J2L_HIP(hipMemcpyAsync(pass_snaps.data(), D + o_snaps, 4 * pass_snaps.size(), hipMemcpyDeviceToHost, st));
+        }
+        // ---- readback ----------------------------------------------------------------------
+        if (o_res - o_sel == h_res - h_sel && o_totals - o_sel == h_tot - h_sel && o_lambda - o_sel == h_lam - h_sel) {
+            J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, o_lambda + 8 * ngroups - o_sel, hipMemcpyDeviceToHost, st));
+        } else {
+            J2L_HIP(hipMemcpyAsync(P + h_sel, D + o_sel, sizeof(Selection) * nb, hipMemcpyDeviceToHost, st));
+            J2L_HIP(hipMemcpyAsync(P + h_res, D + o_res, sizeof(BlockResult) * nb, hipMemcpyDeviceToHost, st));
+            J2L_HIP(hipMemcpyAsync(P + h_tot, D + o_totals, 4 * imgs.size(), hipMemcpyDeviceToHost, st));
+            J2L_HIP(hipMemcpyAsync(P + h_lam, D + o_lambda, 8 * ngroups, hipMemcpyDeviceToHost, st));
+        }
+        J2L_HIP(hipStreamSynchronize(st));
+        t_dwt_ms = ev.ms(0, 1);
+        t_t1_ms = ev.ms(1, 2);
+        t_pcrd_ms = ev.ms(2, 3);
+        sel = (const Selection*)(P + h_sel);
+        res = (const BlockResult*)(P + h_res);
+    };
+    finish();
     if (run != 1) {
         bool pool_full = false;
         for (int b = 0; b < nb && !pool_full; b++) pool_full = res[b].overflow == 2;
         if (pool_full) {
+            // the formed streams overflowed their pool: code with the wave
+            // coder instead (eagerly; the DWT output is still in place)
             run = 1;
-            goto t1_again;
+            coding(1, false);
+            finish();
         }
     }
     last_t1_lanes = run == 2;
@@ -961,7 +1059,13 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         im.pay_off = pay_total;
         pay_total += im.ci.payload_len;
     }
-    const size_t o_pay = L.add(pay_total + 64);
+    // The payload region is rounded up to a granule of an eighth of its
+    // size (a power of two, at least 4 KiB), so that pages of one shape whose
+    // code-streams differ a little in length share an arena layout, and with
+    // it a captured launch sequence.
+    size_t pay_gran = 4096;
+    while (pay_gran * 8 < pay_total) pay_gran *= 2;
+    const size_t o_pay = L.add((pay_total + 64 + pay_gran - 1) / pay_gran * pay_gran);
     int maxL = 0;
     for (auto& im : imgs) {
         const int nc = im.ci.ncomp;
@@ -1108,18 +1212,30 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     pin_.reserve(H.used);
     copy_staged(staged, D, pin_.base, st);
 
-    Events ev;
+    Events& ev = ev_;
+    // Tier-1 is one launch; the inverse DWT levels and the pixel output run
+    // as one graph (or their launches), between the stage events.
     J2L_HIP(hipEventRecord(ev.e[0], st));
     launch_t1_decode((const DecodeJob*)(D + o_jobs), nj, tables_, (const uint8_t*)(D + o_pay), st);
     J2L_HIP(hipEventRecord(ev.e[1], st));
-    for (int k = 0; k < maxL; k++)
-        for (int irrev = 0; irrev <= 1; irrev++) {
-            auto& v = isegs[2 * k + irrev];
-            if (v.empty()) continue;
-            launch_inv_level(irrev != 0, (const DwtSeg*)(D + o_iseg[2 * k + irrev]), (int)v.size(), itiles[2 * k + irrev], st);
+    graphs_.run(st, [&](bool go, std::vector<uint64_t>& sig) {
+        sig.insert(sig.end(), {9, (uint64_t)(uintptr_t)D, (uint64_t)L.used, o_fseg0, o_fseg1, fsegs[0].size(),
+                               fsegs[1].size(), (uint64_t)fblocks[0], (uint64_t)fblocks[1]});
+        for (int k = 0; k < maxL; k++)
+            for (int irrev = 0; irrev <= 1; irrev++) {
+                auto& v = isegs[2 * k + irrev];
+                if (v.empty()) continue;
+                sig.insert(sig.end(), {10, (uint64_t)(2 * k + irrev), o_iseg[2 * k + irrev], v.size(),
+                                       (uint64_t)itiles[2 * k + irrev]});
+                if (go)
+                    launch_inv_level(irrev != 0, (const DwtSeg*)(D + o_iseg[2 * k + irrev]), (int)v.size(),
+                                     itiles[2 * k + irrev], st);
+            }
+        if (go) {
+            launch_finalize((const PixSeg*)(D + o_fseg0), (int)fsegs[0].size(), fblocks[0], st);
+            launch_finalize((const PixSeg*)(D + o_fseg1), (int)fsegs[1].size(), fblocks[1], st);
         }
-    launch_finalize((const PixSeg*)(D + o_fseg0), (int)fsegs[0].size(), fblocks[0], st);
-    launch_finalize((const PixSeg*)(D + o_fseg1), (int)fsegs[1].size(), fblocks[1], st);
+    });
     J2L_HIP(hipEventRecord(ev.e[2], st));
     for (size_t k = 0; k < imgs.size(); k++) {
         auto& im = imgs[k];

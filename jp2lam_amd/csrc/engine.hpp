@@ -2,6 +2,7 @@
 // whole encode/decode batches (one image = a batch of one).
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -68,6 +69,45 @@ struct Arena {
     ~Arena();
 };
 
+struct Events {
+    hipEvent_t e[8];
+    Events() {
+        for (auto& x : e) J2L_HIP(hipEventCreate(&x));
+    }
+    ~Events() {
+        for (auto& x : e) (void)hipEventDestroy(x);
+    }
+    float ms(int a, int b) {
+        float t = 0;
+        J2L_HIP(hipEventElapsedTime(&t, e[a], e[b]));
+        return t;
+    }
+};
+
+// Launch sequences (kernels and device-to-device copies of one stage) run
+// as HIP graphs; the stage events are recorded on the stream between them,
+// since event records captured into a graph did not give usable times.  A sequence is named by its exact signature: every step's
+// kind, arena offsets, counts and grid sizes, and the arena base.  It is
+// captured on its second occurrence and replayed from then on; a batch of a
+// new shape runs eagerly.  Inputs and descriptor tables are copied into the
+// arena before the launch, so a replay codes the new data.
+class GraphCache {
+public:
+    ~GraphCache();
+    void run(hipStream_t st, const std::function<void(bool go, std::vector<uint64_t>& sig)>& seq);
+    void clear();
+
+private:
+    static constexpr size_t kMaxGraphs = 32;
+    std::map<std::vector<uint64_t>, hipGraphExec_t> exec_;
+    std::map<std::vector<uint64_t>, int> seen_;
+};
+
+// Process-wide switch for the graph caches of every engine (default off) and
+// their totals: sequences captured, graph launches.
+void set_graphs(bool on);
+void graph_stats(uint64_t* captures, uint64_t* replays);
+
 class Engine {
 public:
     Engine();
@@ -109,11 +149,14 @@ public:
     double t_dwt_ms = 0, t_t1_ms = 0, t_pcrd_ms = 0, t_host_ms = 0;
     double t_metrics_ms = 0;  // encode_with_psnr reconstruction + reductions
     hipStream_t stream() const { return stream_; }
+    GraphCache& graphs() { return graphs_; }
     std::mutex mu;
 
 private:
     hipStream_t stream_ = nullptr;
     Arena dev_, pin_;
+    Events ev_;
+    GraphCache graphs_;
     uint32_t* tables_ = nullptr;
     std::vector<BlockStat> last_blocks_;
     std::vector<uint32_t> last_snaps_;

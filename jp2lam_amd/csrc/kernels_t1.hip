@@ -1973,6 +1973,13 @@ void launch_t1_hybrid(const BlockJob* jobs, int n, const uint32_t* tabs, const u
 void launch_t1_count(const BlockJob* jobs, int n, const uint32_t* tabs, uint32_t* counts, hipStream_t st) {
     if (n > 0) t1_count_kernel<<<n, 64, 0, st>>>(jobs, tabs, counts);
 }
+__global__ __launch_bounds__(64) void zero_u64_kernel(unsigned long long* __restrict__ p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0;
+}
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t st) {
+    if (n > 0) zero_u64_kernel<<<(n + 63) / 64, 64, 0, st>>>(p, n);
+}
 uint64_t t1_stream_bound(uint32_t mbp) { return stream_bound(mbp); }
 void launch_t1_decode(const DecodeJob* jobs, int n, const uint32_t* tabs, const uint8_t* payload, hipStream_t st) {
     if (n > 0) t1_decode_kernel<<<n, 64, 0, st>>>(jobs, tabs, payload);
