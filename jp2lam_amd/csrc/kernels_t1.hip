@@ -91,7 +91,7 @@ __device__ __forceinline__ uint32_t sc_entry(const Tables& t, uint32_t si) { ret
 struct MqDec {
     uint32_t a, c, ct, bp, cur;  // cur = byte at bp
     uint32_t tab;                // VGPR: transition table
-    uint32_t w0, w1, w2, dbase;
+    uint32_t w0, w1, w2, dbase, lim;  // lim = dbase + 88: slide when bp reaches it
     const uint8_t* data;
     uint32_t len;
 };
@@ -101,14 +101,17 @@ __device__ __forceinline__ uint32_t dec_load64(const MqDec& d, uint32_t base) {
 }
 __device__ __forceinline__ void dec_window_init(MqDec& d) {
     d.dbase = 0;
+    d.lim = 88;
     d.w0 = dec_load64(d, 0);
     d.w1 = dec_load64(d, 64);
     d.w2 = dec_load64(d, 128);
 }
 __device__ __forceinline__ void dec_slide(MqDec& d) {
-    if (d.bp + 40u >= d.dbase + 128u) {
-        d.w0 = d.w1;
-        d.w1 = d.w2;
+    if (__builtin_expect(d.bp >= d.lim, 0)) {
+        d.lim += 64u;
+        // w0 = w1, w1 = w2 in place (written as plain copies, the rotation
+        // made the compiler copy the window registers on every column)
+        asm volatile("v_mov_b32 %0, %1\n\tv_mov_b32 %1, %2" : "+v"(d.w0), "+v"(d.w1) : "v"(d.w2));
         d.dbase += 64u;
         d.w2 = dec_load64(d, d.dbase + 128u);
     }
@@ -118,7 +121,7 @@ __device__ __forceinline__ void dec_slide(MqDec& d) {
 __device__ __forceinline__ void dec_start(MqDec& d) {
     dec_window_init(d);
     d.bp = 0;
-    d.a = 0x8000u;
+    d.a = 0x80000000u;  // 0x8000 << 16
     d.cur = rl(d.w0, 0);
     d.c = d.cur << 16;
     const uint32_t next = rl(d.w0, 1);
@@ -145,18 +148,21 @@ __device__ __forceinline__ void dec_start(MqDec& d) {
 // case that keeps a >= 0x8000 exits early; otherwise the LPS flag becomes a
 // mask and one common tail updates the state, forms the decision and shifts
 // (byte-in only when ct runs out).  M0 holds the context index on entry.
+// The interval a is held shifted up by 16 bits, and the state entries of
+// the decoder carry qe << 16 (dec_entry), so the comparisons with c's high
+// half and the subtraction from it need no shifts, the a >= 0x8000 test is
+// a sign test, and the renormalising shift is the leading-zero count of a.
 __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t ctx) {
     uint32_t bit, e, q, t, k, n0, n1;
     asm volatile(
-        "s_lshr_b32 %[t], %[c], 16\n\t"  // also the M0 wait state
+        "s_nop 0\n\t"                 // the M0 wait state
         "s_movrels_b32 %[e], s80\n\t"
-        "s_and_b32 %[q], %[e], 0xffff\n\t"
+        "s_and_b32 %[q], %[e], 0x7fff0000\n\t"
         "s_sub_u32 %[a], %[a], %[q]\n\t"
-        "s_cmp_lt_u32 %[t], %[q]\n\t"
+        "s_cmp_lt_u32 %[c], %[q]\n\t"   // c >> 16 < qe
         "s_cbranch_scc1 1f\n\t"
-        "s_lshl_b32 %[t], %[q], 16\n\t"
-        "s_sub_u32 %[c], %[c], %[t]\n\t"
-        "s_bitcmp1_b32 %[a], 15\n\t"
+        "s_sub_u32 %[c], %[c], %[q]\n\t"
+        "s_cmp_lt_i32 %[a], 0\n\t"
         "s_cbranch_scc0 7f\n\t"         // renormalising upper path: the one taken branch
         "s_lshr_b32 %[bit], %[e], 31\n\t"   // MPS, a stays >= 0x8000
         "s_branch 9f\n"
@@ -209,16 +215,15 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_cmp_lt_u32 %[a], %[q]\n\t"
         "s_cselect_b32 %[k], -1, 0\n"
         "8:\n\t"                    // k = all ones on LPS, and so is SCC
-        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
+        "s_cselect_b32 %[t], 0x60006, %[f0]\n\t"  // NLPS / NMPS field: width 6 at 6 / 0
         "s_bfe_u32 %[t], %[e], %[t]\n\t"
-        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "v_readlane_b32 %[n1], %[tab], %[t]\n\t"  // used four instructions later
         "s_xor_b32 %[bit], %[e], %[k]\n\t"   // bit 31: MPS ^ LPS = the decision
         "s_lshr_b32 %[bit], %[bit], 31\n\t"
         "s_and_b32 %[n0], %[e], 0x80000000\n\t"
-        "s_xor_b32 %[t], %[t], %[n0]\n\t"   // bit 31: MPS, flipped by a switching LPS
-        "s_movreld_b32 s80, %[t]\n\t"
         "s_flbit_i32_b32 %[t], %[a]\n\t"
-        "s_sub_u32 %[t], %[t], 16\n\t"
+        "s_xor_b32 %[n1], %[n1], %[n0]\n\t"   // bit 31: MPS, flipped by a switching LPS
+        "s_movreld_b32 s80, %[n1]\n\t"
         "s_cmp_gt_u32 %[t], %[ct]\n\t"
         "s_cbranch_scc1 2b\n\t"
         "s_lshl_b32 %[a], %[a], %[t]\n\t"
@@ -227,9 +232,14 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "9:"
         : [a] "+s"(d.a), [c] "+s"(d.c), [ct] "+s"(d.ct), [bp] "+s"(d.bp), [cur] "+s"(d.cur), [m0] "+{m0}"(ctx), [bit] "=&s"(bit), [e] "=&s"(e), [q] "=&s"(q), [t] "=&s"(t), [k] "=&s"(k),
           [n0] "=&s"(n0), [n1] "=&s"(n1)
-        : [tab] "v"(d.tab), [w0] "v"(d.w0), [w1] "v"(d.w1), [db] "s"(d.dbase), [f16] "s"(0x60010u)
+        : [tab] "v"(d.tab), [w0] "v"(d.w0), [w1] "v"(d.w1), [db] "s"(d.dbase), [f0] "s"(0x60000u)
         : "scc");
     return bit;
+}
+// A state-table lane (qe | nmps << 16 | nlps << 22 | flip-MPS << 31) in the
+// decoder's layout: qe << 16 | nmps | nlps << 6 | flip-MPS << 31.
+__device__ __forceinline__ uint32_t dec_entry(uint32_t v) {
+    return ((v & 0xffffu) << 16) | ((v >> 16) & 63u) | (((v >> 22) & 63u) << 6) | (v & 0x80000000u);
 }
 
 // ---- shared scan machinery ------------------------------------------------------
@@ -1623,11 +1633,77 @@ __device__ __forceinline__ uint32_t dec_sign(Coder<true>& cd, const Tables& t, u
     return cd.code(t, e & 15u, 0) ^ (e >> 4);
 }
 
-// Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).  The
-// column word also carries the rows that can be coded (insignificant with a
-// significant neighbour at the start of the stripe); the scan visits only
-// those, plus the row below one that becomes significant and, in the column
-// after one with new significance, the rows next to it.
+// Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).
+// Lane-parallel at the start of each stripe, every column gets one word:
+// the ZC contexts of its four rows from the state at that point (nibbles
+// 0..3), the rows that can be coded (insignificant with a significant
+// neighbour, bits 16..19) and the insignificant rows (bits 20..23).  The
+// scalar scan then reads one word per column and one nibble per row; the
+// state it changes (rows that become significant, their column's and the
+// next column's words) is handled on the path of a significant decision
+// alone.  The scan visits the rows of the word, the row below one that
+// becomes significant, and, in the column after one with new significance,
+// the rows next to it.
+__device__ __forceinline__ uint32_t ctx4_lane(const Tables& t, uint32_t X) {
+    return zc_lane(t, X & 0x1ffu) | (zc_lane(t, (X >> 3) & 0x1ffu) << 4) | (zc_lane(t, (X >> 6) & 0x1ffu) << 8) |
+           (zc_lane(t, (X >> 9) & 0x1ffu) << 12);
+}
+__device__ __forceinline__ uint32_t ctx4_scalar(const Tables& t, uint32_t Xc) {
+    return zc_ctx(t, Xc & 0x1ffu) | (zc_ctx(t, (Xc >> 3) & 0x1ffu) << 4) | (zc_ctx(t, (Xc >> 6) & 0x1ffu) << 8) |
+           (zc_ctx(t, (Xc >> 9) & 0x1ffu) << 12);
+}
+// Scalar state of one stripe's scan.  fx: the column whose west neighbour
+// gained significance (the last column with a significant decision, plus
+// one); acc: the column that Xc, Yc (neighbourhood words with this pass's
+// changes), newC, newCs (rows that became significant and their signs), nX,
+// nY (their bits in the next column's words) and vadd (rows visited beyond
+// the word's) belong to.
+struct Scan {
+    uint32_t fx, acc, Xc, Yc, nX, nY, newC, newCs, vadd;
+};
+// The scan state starts from SGPR constants written by one asm: constants
+// the compiler materialises with v_mov could reach the MQ asm operands
+// through the loop phis as VGPRs.
+__device__ __forceinline__ Scan scan_init() {
+    uint32_t z, none;
+    asm volatile("s_mov_b32 %0, 0\n\ts_mov_b32 %1, -1" : "=s"(z), "=s"(none));
+    return Scan{none, none, z, z, z, z, z, z, z};
+}
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+// A value the compiler cannot see through (an empty asm), so that work on
+// the path of a significant decision is not hoisted to every column.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+// Start the accumulators of column x: its words, with the west column's new
+// significance when that column was the previous one with a significant
+// decision.
+__device__ __forceinline__ void scan_column(Scan& z, uint32_t X, uint32_t Y, uint32_t x) {
+    const bool west = z.fx == x;
+    z.Xc = uni(rl(X, x) | (west ? z.nX : 0u));
+    z.Yc = uni(rl(Y, x) | (west ? z.nY : 0u));
+    z.acc = x;
+    z.newC = z.newCs = z.nX = z.nY = z.vadd = uni(0u);
+}
+// A row of column x became significant: its sign, the words, and the
+// context of the row below (its north neighbour is now significant).
+__device__ __forceinline__ void scan_significant(Scan& z, Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t ci,
+                                                 uint32_t x, uint32_t& P) {
+    const uint32_t nb = (z.Xc >> (3u * ci)) & 0x1ffu;
+    const uint32_t sg = dec_sign(cd, t, sc2, nb, z.Yc, ci);
+    z.Xc |= 1u << (3u * ci + 4u);
+    z.Yc |= sg << (3u * ci + 4u);
+    z.nX |= 1u << (3u * ci + 3u);
+    z.nY |= sg << (3u * ci + 3u);
+    z.newC |= 1u << ci;
+    z.newCs |= sg << ci;
+    if (ci < 3u) {
+        const uint32_t c1 = ci + 1u;
+        P = (P & ~(15u << (4u * c1))) | (zc_ctx(t, (z.Xc >> (3u * c1)) & 0x1ffu) << (4u * c1));
+    }
+    z.fx = x + 1u;
+}
 __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
                             int w, int h) {
     const int lane = lid();
@@ -1640,60 +1716,56 @@ __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         const uint32_t o6 = from_west(sig6) | sig6 | from_east(sig6);
         const uint32_t insig4 = (~(sig6 >> 1)) & rmask;
         const uint32_t cand = insig4 & (o6 | (o6 >> 1) | (o6 >> 2));
-        const uint32_t PX = X | (cand << 18) | (insig4 << 22);
+        const uint32_t PL = ctx4_lane(t, X) | (cand << 16) | (insig4 << 20);
         uint64_t M = ballot(lane < w && cand != 0);
-        uint32_t upd = 0, nWX = 0, nWY = 0, newW = 0;
-        int x = M ? __builtin_ctzll(M) : 64;
-        while (x < w) {
-            M &= ~(1ull << x);
+        uint32_t upd = 0;
+        Scan z = scan_init();
+        while (M) {
+            const uint32_t x = (uint32_t)__builtin_ctzll(M);
+            M &= M - 1ull;
             cd.column();
-            const uint32_t Pc = rl(PX, x);
-            const uint32_t insig = Pc >> 22;
-            uint32_t Xc = (Pc & 0x3ffffu) | nWX;
-            uint32_t visit = ((Pc >> 18) & 15u) | (insig & (newW | (newW << 1) | (newW >> 1)));
-            uint32_t Yc = 0;
-            bool have_y = false;
-            uint32_t newC = 0, newCs = 0, visC = 0, nX = 0, nY = 0;
+            uint32_t P = rl(PL, x);
+            uint32_t visit = (P >> 16) & 15u;
+            if (__builtin_expect(z.fx == x, 0)) {
+                // the west column gained significance: its bits, the
+                // contexts, and the rows that now have a significant neighbour
+                const uint32_t newW = z.newC;
+                scan_column(z, X, Y, x);
+                P = (P & ~0xffffu) | ctx4_scalar(t, z.Xc);
+                z.vadd = (P >> 20) & (newW | (newW << 1) | (newW >> 1)) & ~visit & 15u;
+                visit |= z.vadd;
+                if (z.vadd) upd = wl(upd, z.vadd << 8, opaque(x));
+                z.fx = 0xffffffffu;
+            }
             while (visit) {
                 const uint32_t ci = (uint32_t)__builtin_ctz(visit);
                 visit &= visit - 1u;
-                const uint32_t nb = (Xc >> (3u * ci)) & 0x1ffu;
-                const uint32_t bit = cd.code(t, zc_ctx(t, nb), 0);
-                visC |= 1u << ci;
-                if (bit) {
-                    if (!have_y) {
-                        Yc = rl(Y, x) | nWY;
-                        have_y = true;
-                    }
-                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, ci);
-                    Xc |= 1u << (3u * ci + 4u);
-                    Yc |= sg << (3u * ci + 4u);
-                    nX |= 1u << (3u * ci + 3u);
-                    nY |= sg << (3u * ci + 3u);
-                    newC |= 1u << ci;
-                    newCs |= sg << ci;
-                    visit |= insig & (2u << ci);  // the row below gains a significant neighbour
+                if (__builtin_expect(cd.code(t, (P >> (4u * ci)) & 15u, 0), 0)) {
+                    if (z.acc != x) scan_column(z, X, Y, x);
+                    scan_significant(z, cd, t, sc2, ci, x, P);
+                    // the row below now has a significant neighbour
+                    const uint32_t below = (P >> 20) & (2u << ci) & ~(P >> 16);
+                    visit |= (P >> 20) & (2u << ci);
+                    z.vadd |= below;
+                    const uint32_t xs = opaque(x);
+                    upd = wl(upd, z.newC | (z.newCs << 4) | (z.vadd << 8), xs);
+                    if (xs + 1u < (uint32_t)w) M |= 1ull << (xs + 1u);
                 }
             }
-            upd = wl(upd, newC | (newCs << 4) | (visC << 8), x);
-            int nx = M ? __builtin_ctzll(M) : 64;
-            if (newC && x + 1 < w) nx = x + 1;
-            const bool next = nx == x + 1;
-            nWX = next ? nX : 0u;
-            nWY = next ? nY : 0u;
-            newW = next ? newC : 0u;
-            x = nx;
         }
         st.sig |= (uint64_t)(upd & 15u) << (4 * s);
         st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
-        st.vis |= (uint64_t)((upd >> 8) & 15u) << (4 * s);
+        st.vis |= (uint64_t)(((lane < w ? cand : 0u) | (upd >> 8)) & 15u) << (4 * s);
         plane |= (uint64_t)(upd & 15u) << (4 * s);
     }
 }
 
 // Cleanup pass with run-length mode on full clean stripes (t1.rs:977-1197,
 // decode/t1.rs:317-446).  Every row neither significant nor visited is
-// coded; the column word carries those rows.
+// coded.  The column word holds the row contexts (nibbles 0..3), those rows
+// (bits 16..19) and whether the column starts in run mode (bit 20: four
+// rows, nothing significant around them, nothing visited); a column after
+// one with new significance loses run mode.
 __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
                             int w, int h) {
     const int lane = lid();
@@ -1704,70 +1776,48 @@ __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         const uint32_t sig6 = win6(st.sig, s), sgn6 = win6(st.sgn, s);
         const uint32_t vis4 = (uint32_t)(st.vis >> (4 * s)) & 15u;
         const uint32_t X = interleave3(spr, sig6), Y = interleave3(spr, sgn6);
-        // static part of stripe_is_clean (flags.rs:202-215): own rows neither
-        // significant nor visited, no significant neighbour in W/own/E windows
+        // static part of stripe_is_clean (flags.rs:202-215)
         const uint32_t sclean = (nrows == 4 && X == 0 && vis4 == 0) ? 1u : 0u;
         const uint32_t todo = ~((sig6 >> 1) | vis4) & rmask;
-        const uint32_t PX = X | (todo << 18) | (sclean << 22);
+        const uint32_t PL = ctx4_lane(t, X) | (todo << 16) | (sclean << 20);
         uint64_t M = ballot(lane < w && todo != 0);
-        uint32_t upd = 0, nWX = 0, nWY = 0;
-        int prevx = -2;
+        uint32_t upd = 0;
+        Scan z = scan_init();
         while (M) {
-            const int x = __builtin_ctzll(M);
-            M &= M - 1;
-            if (x != prevx + 1) {
-                nWX = 0;
-                nWY = 0;
-            }
+            const uint32_t x = (uint32_t)__builtin_ctzll(M);
+            M &= M - 1ull;
             cd.column();
-            const uint32_t Pc = rl(PX, x);
-            uint32_t Xc = (Pc & 0x3ffffu) | nWX;
-            uint32_t visit = (Pc >> 18) & 15u;
-            uint32_t Yc = 0;
-            bool have_y = false;
-            uint32_t newC = 0, newCs = 0, nX = 0, nY = 0;
-            if (((Pc >> 22) & 1u) && nWX == 0) {  // run mode
-                if (!cd.code(t, CTX_AGG, 0)) {
+            uint32_t P = rl(PL, x);
+            uint32_t visit = (P >> 16) & 15u;
+            if (__builtin_expect(z.fx == x, 0)) {
+                scan_column(z, X, Y, x);
+                P = (P & ~0x10ffffu) | ctx4_scalar(t, z.Xc);
+                z.fx = 0xffffffffu;
+            }
+            if (P & 0x100000u) {  // run mode
+                if (__builtin_expect(!cd.code(t, CTX_AGG, 0), 1)) {
                     visit = 0;
                 } else {
                     const uint32_t hi = cd.code(t, CTX_UNI, 0);
                     const uint32_t lo = cd.code(t, CTX_UNI, 0);
                     const uint32_t rlen = (hi << 1) | lo;
-                    // a clean stripe has no significant neighbour: sign index 0
-                    const uint32_t sg = dec_sign(cd, t, sc2, 0u, 0u, 0u);
-                    Xc |= 1u << (3u * rlen + 4u);
-                    Yc = sg << (3u * rlen + 4u);
-                    have_y = true;
-                    nX = 1u << (3u * rlen + 3u);
-                    nY = sg << (3u * rlen + 3u);
-                    newC = 1u << rlen;
-                    newCs = sg << rlen;
+                    scan_column(z, X, Y, x);
+                    // a clean stripe has no significant neighbour: sign
+                    // index 0 (its contexts are 0 as well)
+                    scan_significant(z, cd, t, sc2, rlen, x, P);
                     visit &= ~((2u << rlen) - 1u);  // rows after rlen
+                    upd = wl(upd, z.newC | (z.newCs << 4), opaque(x));
                 }
             }
             while (visit) {
                 const uint32_t ci = (uint32_t)__builtin_ctz(visit);
                 visit &= visit - 1u;
-                const uint32_t nb = (Xc >> (3u * ci)) & 0x1ffu;
-                const uint32_t bit = cd.code(t, nb ? zc_ctx(t, nb) : (uint32_t)CTX_ZC, 0);
-                if (bit) {
-                    if (!have_y) {
-                        Yc = rl(Y, x) | nWY;
-                        have_y = true;
-                    }
-                    const uint32_t sg = dec_sign(cd, t, sc2, nb, Yc, ci);
-                    Xc |= 1u << (3u * ci + 4u);
-                    Yc |= sg << (3u * ci + 4u);
-                    nX |= 1u << (3u * ci + 3u);
-                    nY |= sg << (3u * ci + 3u);
-                    newC |= 1u << ci;
-                    newCs |= sg << ci;
+                if (__builtin_expect(cd.code(t, (P >> (4u * ci)) & 15u, 0), 0)) {
+                    if (z.acc != x) scan_column(z, X, Y, x);
+                    scan_significant(z, cd, t, sc2, ci, x, P);
+                    upd = wl(upd, z.newC | (z.newCs << 4), opaque(x));
                 }
             }
-            if (newC) upd = wl(upd, newC | (newCs << 4), x);
-            nWX = nX;
-            nWY = nY;
-            prevx = x;
         }
         st.sig |= (uint64_t)(upd & 15u) << (4 * s);
         st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
@@ -1905,9 +1955,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
         Coder<true> cd;
         cd.m.data = payload + J.data_off;
         cd.m.len = J.data_len;
-        cd.m.tab = t.mq;
+        cd.m.tab = dec_entry(t.mq);
         // initial states (mq/mod.rs:195-200): UNI 46, AGG 3, ZC 4, others 0; MPS 0
-        states_init((uint32_t)__shfl((int)t.mq, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64));
+        states_init((uint32_t)__shfl((int)cd.m.tab, lane == CTX_UNI ? 46 : lane == CTX_AGG ? 3 : lane == CTX_ZC ? 4 : 0, 64));
         dec_start(cd.m);
         Col st = {0, 0, 0, 0};
         const int mbp = (int)J.mb - (int)J.zbp;
