@@ -551,6 +551,7 @@ jp2lam_status jp2g_forward_dwt(const uint8_t* pixels, uint32_t width, uint32_t h
                 s.in_stride = k == 0 ? (int)width : (int)rw[Lv - k];
                 s.plane = (uint64_t)(uintptr_t)P;
                 s.pstride = (int)width;
+                s.hx = rw[Lv - k - 1];  // the plain Mallat layout
                 const bool last = k == Lv - 1;
                 s.out = (uint64_t)(uintptr_t)(last ? P : ll + (k & 1) * llhalf);
                 s.out_stride = last ? (int)width : (int)rw[Lv - k - 1];
@@ -607,6 +608,7 @@ jp2lam_status jp2g_inverse_dwt(const void* planes, uint32_t width, uint32_t heig
                 s.in_stride = k == 0 ? (int)width : (int)rw[r - 1];
                 s.plane = (uint64_t)(uintptr_t)Pc;
                 s.pstride = (int)width;
+                s.hx = rw[r - 1];  // the plain Mallat layout
                 const bool last = k == levels - 1;
                 s.out = (uint64_t)(uintptr_t)(last ? Oc : ll + (k & 1) * llhalf);
                 s.out_stride = last ? (int)width : (int)rw[r];

@@ -225,6 +225,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         int idx;
         Plan plan;
         std::vector<uint32_t> rw, rh;
+        PlaneLayout pl;  // coefficient plane columns and stride
         size_t n;
         int kind;
         const void* src;
@@ -262,6 +263,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                 }
             }
             resolution_ladder(it.width, it.height, im.plan.levels, im.rw, im.rh);
+            im.pl = plane_layout(it.width, im.plan.levels, true);
             // backend.rs:371-421: metrics cover gray and sRGB only
             if (it.metrics && !im.plan.lossless && it.colorspace != GRAY && it.colorspace != SRGB)
                 throw Error(ENCODE_FAILED, "quality metrics not implemented for this colorspace configuration");
@@ -288,10 +290,10 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         if (im.kind == SRC_HOST_U8) im.in_bytes = im.n * ch;
         else if (im.kind == SRC_HOST_I32) im.in_bytes = im.n * ch * 4;
         if (im.in_bytes) im.in_off = L.add(im.in_bytes);
-        const size_t llw = (p.width + 1) / 2, llh = (p.height + 1) / 2;
+        const size_t llw = ll_stride((p.width + 1) / 2), llh = (p.height + 1) / 2;
         for (int c = 0; c < nc; c++) {
             im.x_off[c] = L.add(im.n * 4);
-            im.p_off[c] = L.add(im.n * 4);
+            im.p_off[c] = L.add((size_t)im.pl.stride * p.height * 4);
             im.ll_off[c][0] = L.add(llw * llh * 4);
             im.ll_off[c][1] = L.add(llw * llh * 4);
         }
@@ -323,8 +325,8 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                         for (int bx = rc.x0; bx < rc.x1; bx += p.cbw) {
                             BlockJob j;
                             j.plane = 0;  // patched after the arena is placed
-                            j.stride = p.width;
-                            j.x0 = (uint16_t)bx;
+                            j.stride = im.pl.stride;
+                            j.x0 = (uint16_t)plane_x(im.pl, rc, r, b, bx);
                             j.y0 = (uint16_t)by;
                             j.w = (uint8_t)(std::min(bx + p.cbw, rc.x1) - bx);
                             j.h = (uint8_t)(std::min(by + p.cbh, rc.y1) - by);
@@ -439,12 +441,13 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
             for (int c = 0; c < p.ncomp; c++) {
                 DwtSeg s{};
                 s.in = k == 0 ? im.x_off[c] : im.ll_off[c][(k - 1) & 1];
-                s.in_stride = k == 0 ? (int)p.width : (int)rw;
+                s.in_stride = k == 0 ? (int)p.width : (int)ll_stride(rw);
                 s.plane = im.p_off[c];
-                s.pstride = (int)p.width;
+                s.pstride = (int)im.pl.stride;
+                s.hx = im.pl.hx[Lv - k];
                 const bool last = k == Lv - 1;
                 s.out = last ? im.p_off[c] : im.ll_off[c][k & 1];
-                s.out_stride = last ? (int)p.width : (int)im.rw[Lv - k - 1];
+                s.out_stride = last ? (int)im.pl.stride : (int)ll_stride(im.rw[Lv - k - 1]);
                 s.rw = (int)rw;
                 s.rh = (int)rh;
                 s.tiles_x = (int)((rw + kTile - 1) / kTile);
@@ -490,12 +493,13 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
             for (int c = 0; c < p.ncomp; c++) {
                 DwtSeg s{};
                 s.in = k == 0 ? im.p_off[c] : im.ll_off[c][(k - 1) & 1];
-                s.in_stride = k == 0 ? (int)p.width : (int)im.rw[r - 1];
+                s.in_stride = k == 0 ? (int)im.pl.stride : (int)ll_stride(im.rw[r - 1]);
                 s.plane = im.p_off[c];
-                s.pstride = (int)p.width;
+                s.pstride = (int)im.pl.stride;
+                s.hx = im.pl.hx[r];
                 const bool last = k == Lv - 1;
                 s.out = last ? im.x_off[c] : im.ll_off[c][k & 1];
-                s.out_stride = last ? (int)p.width : (int)im.rw[r];
+                s.out_stride = last ? (int)p.width : (int)ll_stride(im.rw[r]);
                 s.rw = (int)im.rw[r];
                 s.rh = (int)im.rh[r];
                 s.tiles_x = (s.rw + kTile - 1) / kTile;
@@ -863,9 +867,11 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                 bs.band = j.band;
                 bs.comp = blk_comp[im.blk_first + b];
                 bs.res = blk_res[im.blk_first + b];
-                bs.x0 = j.x0;
+                // Mallat column (the plane stores the detail bands at
+                // im.pl.hx[res], aligned)
+                bs.x0 = j.band == HL || j.band == HH ? j.x0 - im.pl.hx[bs.res] + im.rw[bs.res - 1] : j.x0;
                 bs.y0 = j.y0;
-                bs.x1 = (uint16_t)(j.x0 + j.w);
+                bs.x1 = (uint16_t)(bs.x0 + j.w);
                 bs.y1 = (uint16_t)(j.y0 + j.h);
                 bs.nbytes = r.nbytes;
                 bs.body_len = sl.body_len;
@@ -1030,6 +1036,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         CodestreamInfo ci;
         std::vector<DecBlock> blocks;
         std::vector<uint32_t> rw, rh;
+        PlaneLayout pl;  // coefficient plane columns and stride
         size_t n;
         size_t pay_off, p_off[3], ll_off[3][2], o_off[3], u8_off;
         uint32_t job_first;
@@ -1042,6 +1049,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
             parse_for_decode(items[i].bytes, items[i].len, im.jp, im.ci, im.blocks);
             im.n = (size_t)im.ci.width * im.ci.height;
             resolution_ladder(im.ci.width, im.ci.height, im.ci.levels, im.rw, im.rh);
+            im.pl = plane_layout(im.ci.width, im.ci.levels, false);
             if (items[i].target == DST_DEV_U8 && items[i].dev_cap < im.n * im.ci.ncomp)
                 throw Error(IO, "device output buffer too small");
             imgs.push_back(std::move(im));
@@ -1071,7 +1079,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         const int nc = im.ci.ncomp;
         const size_t llw = (im.ci.width + 1) / 2, llh = (im.ci.height + 1) / 2;
         for (int c = 0; c < nc; c++) {
-            im.p_off[c] = L.add(im.n * 4);
+            im.p_off[c] = L.add((size_t)im.pl.stride * im.ci.height * 4);
             im.ll_off[c][0] = L.add(llw * llh * 4);
             im.ll_off[c][1] = L.add(llw * llh * 4);
             im.o_off[c] = L.add(im.n * 4);
@@ -1083,8 +1091,9 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         for (auto& b : im.blocks) {
             DecodeJob j{};
             j.plane = (uint64_t)b.comp;  // rebased below
-            j.stride = im.ci.width;
-            j.x0 = (uint16_t)b.r.x0;
+            j.stride = im.pl.stride;
+            const bool detail_x = b.band == HL || b.band == HH;
+            j.x0 = (uint16_t)(detail_x ? im.pl.hx[b.res] + (uint32_t)b.r.x0 - im.rw[b.res - 1] : (uint32_t)b.r.x0);
             j.y0 = (uint16_t)b.r.y0;
             j.w = (uint8_t)(b.r.x1 - b.r.x0);
             j.h = (uint8_t)(b.r.y1 - b.r.y0);
@@ -1112,9 +1121,10 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
             for (int c = 0; c < im.ci.ncomp; c++) {
                 DwtSeg s{};
                 s.in = k == 0 ? im.p_off[c] : im.ll_off[c][(k - 1) & 1];
-                s.in_stride = k == 0 ? (int)im.ci.width : (int)im.rw[r - 1];
+                s.in_stride = k == 0 ? (int)im.pl.stride : (int)im.rw[r - 1];
                 s.plane = im.p_off[c];
-                s.pstride = (int)im.ci.width;
+                s.pstride = (int)im.pl.stride;
+                s.hx = im.pl.hx[r];
                 const bool last = k == Lv - 1;
                 s.out = last ? im.o_off[c] : im.ll_off[c][k & 1];
                 s.out_stride = last ? (int)im.ci.width : (int)im.rw[r];

@@ -173,6 +173,23 @@ Rect band_rect(uint32_t w, uint32_t h, int levels, int res, int band) {
     return {lw, lh, fw, fh};
 }
 
+PlaneLayout plane_layout(uint32_t w, int levels, bool aligned) {
+    PlaneLayout pl;
+    std::vector<uint32_t> rw, rh;
+    resolution_ladder(w, 1, levels, rw, rh);
+    uint32_t end = rw[0];  // columns used by resolutions below r
+    for (int r = 1; r <= levels; r++) {
+        pl.hx[r] = (end + 31u) & ~31u;
+        end = pl.hx[r] + (rw[r] - rw[r - 1]);
+    }
+    pl.stride = levels ? (end + 31u) & ~31u : w;
+    if (!aligned || pl.stride > 65535u) {
+        for (int r = 1; r <= levels; r++) pl.hx[r] = rw[r - 1];
+        pl.stride = w;
+    }
+    return pl;
+}
+
 // ---- Tier-1 tables ------------------------------------------------------------
 namespace {
 // ISO/IEC 15444-1 Table C.2: Qe, next-MPS, next-LPS, switch.

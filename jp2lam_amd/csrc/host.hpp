@@ -50,6 +50,29 @@ struct Rect {
 };
 void resolution_ladder(uint32_t w, uint32_t h, int levels, std::vector<uint32_t>& rw, std::vector<uint32_t>& rh);
 Rect band_rect(uint32_t w, uint32_t h, int levels, int res, int band);
+// Storage of a component's Mallat plane on the device.  The bands keep
+// their Mallat rows; the HL and HH bands of resolution r start at column
+// hx[r].  Aligned (the encoder): hx[r] is a multiple of 32 samples (128
+// bytes) past the end of the lower resolutions' columns and the row stride
+// a multiple of 32 too, so every code-block row starts on a 128-byte line:
+// with the plain Mallat columns a 64-sample block row spans three lines and
+// the Tier-1 reads fetch 1.5x their bytes.  Plain (the decoder): the Mallat
+// columns, stride w; the inverse DWT reads each band row from two samples
+// before a tile's start, which on an aligned band spans one more line.  The
+// aligned form falls back to the plain one when its rows would not fit the
+// 16-bit block offsets.
+struct PlaneLayout {
+    uint32_t stride = 0;
+    uint32_t hx[33] = {};
+};
+PlaneLayout plane_layout(uint32_t w, int levels, bool aligned);
+// Storage column of the sample at Mallat column x of band `band` at
+// resolution `res` (band_rect's coordinates).
+inline uint32_t plane_x(const PlaneLayout& pl, const Rect& rc, int res, int band, int x) {
+    return band == HL || band == HH ? pl.hx[res] + (uint32_t)(x - rc.x0) : (uint32_t)x;
+}
+// Row stride of an LL buffer of width w: a multiple of 32 samples.
+inline uint32_t ll_stride(uint32_t w) { return (w + 31u) & ~31u; }
 
 // Kernel lookup tables (MQ states, ZC/SC contexts) in the per-lane layout
 // the Tier-1 kernels read: 64 MQ words, 4 x 64 ZC words, 64 SC words.

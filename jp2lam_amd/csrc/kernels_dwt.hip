@@ -199,7 +199,8 @@ __global__ __launch_bounds__(1024) void fwd_level_kernel(const DwtSeg* __restric
         for (int ly = threadIdx.x; ly < TLH; ly += blockDim.x) s[ly][HALO] = __fmul_rn(s[ly][HALO], INV_K97);
     }
     __syncthreads();
-    const int snw = (rw + 1) >> 1, snh = (rh + 1) >> 1;
+    // HL and HH start at plane column S.hx, LH and HH at Mallat row snh
+    const int snh = (rh + 1) >> 1, bhx = (int)S.hx;
     for (int idx = threadIdx.x; idx < TY * TX; idx += blockDim.x) {
         int oy = idx / TX, ox = idx % TX;
         int gy = ty0 + oy, gx = tx0 + ox;
@@ -207,10 +208,10 @@ __global__ __launch_bounds__(1024) void fwd_level_kernel(const DwtSeg* __restric
         T v = s[HALO + oy][HALO + ox];
         int hy = gy >> 1, hx = gx >> 1;
         if (gy & 1) {
-            if (gx & 1) plane[(size_t)(snh + hy) * S.pstride + snw + hx] = v;
+            if (gx & 1) plane[(size_t)(snh + hy) * S.pstride + bhx + hx] = v;
             else plane[(size_t)(snh + hy) * S.pstride + hx] = v;
         } else {
-            if (gx & 1) plane[(size_t)hy * S.pstride + snw + hx] = v;
+            if (gx & 1) plane[(size_t)hy * S.pstride + bhx + hx] = v;
             else ll_out[(size_t)hy * S.out_stride + hx] = v;
         }
     }
@@ -228,14 +229,14 @@ __global__ __launch_bounds__(1024) void inv_level_kernel(const DwtSeg* __restric
     const T* __restrict__ ll_in = (const T*)S.in;
     const T* __restrict__ plane = (const T*)S.plane;
     T* __restrict__ out = (T*)S.out;
-    const int snw = (rw + 1) >> 1, snh = (rh + 1) >> 1;
+    const int snh = (rh + 1) >> 1, bhx = (int)S.hx;
     for (int idx = threadIdx.x; idx < TLH * TLW; idx += blockDim.x) {
         int ly = idx / TLW, lx = idx % TLW;
         int my = sym(rh, ty0 - HALO + ly), mx = sym(rw, tx0 - HALO + lx);
         int hy = my >> 1, hx = mx >> 1;
         T v;
-        if (my & 1) v = plane[(size_t)(snh + hy) * S.pstride + ((mx & 1) ? snw + hx : hx)];
-        else if (mx & 1) v = plane[(size_t)hy * S.pstride + snw + hx];
+        if (my & 1) v = plane[(size_t)(snh + hy) * S.pstride + ((mx & 1) ? bhx + hx : hx)];
+        else if (mx & 1) v = plane[(size_t)hy * S.pstride + bhx + hx];
         else v = ll_in[(size_t)hy * S.in_stride + hx];
         s[ly][lx] = v;
     }

@@ -37,7 +37,8 @@ constexpr int kMaxPasses = 96;     // 1 + 3 * (31 - 1) rounded up
 constexpr int kMaxDecodeLevels = 32;
 
 // One code-block job for the Tier-1 kernels.  Coordinates are in the
-// component's Mallat plane (layout.rs:37-173).
+// component's coefficient plane: the Mallat layout of layout.rs:37-173, with
+// the detail bands' columns placed as host.hpp's plane_layout says.
 struct BlockJob {
     uint64_t plane;     // device address of the coefficient plane (f32 lossy / i32 lossless)
     uint32_t stride;    // plane row stride in elements
@@ -103,7 +104,8 @@ struct DwtSeg {
     uint64_t out;      // forward: LL output; inverse: interleaved output
     int32_t in_stride, pstride, out_stride;
     int32_t rw, rh, tiles_x;
-    uint32_t tile_start, pad;
+    uint32_t tile_start;
+    uint32_t hx;       // plane column of this level's HL and HH bands
 };
 
 // Pixel launch segment: one image component (prepare) or one image
