@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
+#include <mutex>
 #include <thread>
 #include <chrono>
 #include <cmath>
@@ -1041,10 +1043,14 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         size_t pay_off, p_off[3], ll_off[3][2], o_off[3], u8_off;
         uint32_t job_first;
     };
-    std::vector<Img> imgs;
-    for (size_t i = 0; i < items.size(); i++) {
+    // Container, codestream and packet headers: each stream on its own, so a
+    // batch of several spreads them over host threads (decode/t2.rs parses
+    // one stream at a time).
+    std::vector<Img> parsed(items.size());
+    std::vector<char> ok(items.size(), 0);
+    auto parse_one = [&](size_t i) {
         try {
-            Img im;
+            Img& im = parsed[i];
             im.idx = (int)i;
             parse_for_decode(items[i].bytes, items[i].len, im.jp, im.ci, im.blocks);
             im.n = (size_t)im.ci.width * im.ci.height;
@@ -1052,12 +1058,40 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
             im.pl = plane_layout(im.ci.width, im.ci.levels, false);
             if (items[i].target == DST_DEV_U8 && items[i].dev_cap < im.n * im.ci.ncomp)
                 throw Error(IO, "device output buffer too small");
-            imgs.push_back(std::move(im));
+            ok[i] = 1;
         } catch (const Error& e) {
             outs[i].status = e.status;
             outs[i].message = e.what();
         }
+    };
+    const size_t nthr = items.size() < 4 ? 1 : std::min<size_t>({items.size(), 8, std::max(1u, std::thread::hardware_concurrency())});
+    if (nthr <= 1) {
+        for (size_t i = 0; i < items.size(); i++) parse_one(i);
+    } else {
+        // other exceptions (out of memory) fail the whole call, as they do
+        // when the streams are parsed one after another
+        std::atomic<size_t> next{0};
+        std::exception_ptr fatal;
+        std::mutex fatal_mu;
+        auto work = [&] {
+            for (size_t i; (i = next.fetch_add(1)) < items.size();) {
+                try {
+                    parse_one(i);
+                } catch (...) {
+                    std::lock_guard<std::mutex> g(fatal_mu);
+                    if (!fatal) fatal = std::current_exception();
+                }
+            }
+        };
+        std::vector<std::thread> ts;
+        for (size_t t = 1; t < nthr; t++) ts.emplace_back(work);
+        work();
+        for (auto& t : ts) t.join();
+        if (fatal) std::rethrow_exception(fatal);
     }
+    std::vector<Img> imgs;
+    for (size_t i = 0; i < items.size(); i++)
+        if (ok[i]) imgs.push_back(std::move(parsed[i]));
     if (imgs.empty()) return outs;
 
     Layout L;
