@@ -1603,15 +1603,13 @@ __global__ __launch_bounds__(64) void t1_hybrid_kernel(int nb, const BlockJob* _
 }
 
 // ---- decoder significance propagation and cleanup passes ----------------------
-// The scan keeps, per column of the current stripe, X: the significance of
-// the west, own and east columns interleaved, bits 3k, 3k + 1, 3k + 2 for
-// window row k (row 4s - 1 + k), and Y: their signs in the same places.
-// Row ci's neighbourhood is then (X >> 3ci) & 0x1ff, bits NW N NE W C E SW S
-// SE: the ZC table index of tier1/consts.rs:1-27 with the own bit (C) in the
+// At the start of a stripe, lane x gets X: the significance of columns
+// x - 1, x and x + 1 interleaved, bits 3k, 3k + 1, 3k + 2 for window row k
+// (row 4s - 1 + k), and Y: their signs in the same places.  Row ci's
+// neighbourhood is then (X >> 3ci) & 0x1ff, bits NW N NE W C E SW S SE: the
+// ZC table index of tier1/consts.rs:1-27 with the own bit (C) in the
 // middle; its N W E S significance (bits 1 3 5 7) and the matching sign bits
-// of Y shifted down by one form the index of the second sign table.  Both
-// words come lane-parallel from the state at the start of the stripe; the
-// scan adds what it decodes (own rows below, the next column's west bits).
+// of Y shifted down by one form the index of the second sign table.
 
 // spr lane v = v (six bits) with bit k moved to bit 3k.
 __device__ __forceinline__ uint32_t spread_lut() {
@@ -1624,90 +1622,69 @@ __device__ __forceinline__ uint32_t interleave3(uint32_t spr, uint32_t own6) {
     return from_west(sp) | (sp << 1) | (from_east(sp) << 2);
 }
 __device__ __forceinline__ uint32_t sc2_entry(uint32_t sc2, uint32_t si) { return (rl(sc2, si >> 2) >> ((si & 3u) * 8u)) & 255u; }
-// The sign decision of a row that became significant (decode/t1.rs sign
-// coding with the context of tier1/helpers.rs:53-89); returns the sign.
-__device__ __forceinline__ uint32_t dec_sign(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t nb, uint32_t Yc,
-                                            uint32_t ci) {
-    const uint32_t si = (nb & 0xAAu) | (((Yc >> (3u * ci)) & 0xAAu) >> 1);
+
+// Significance propagation and cleanup scans.  The scan of a stripe keeps,
+// in two scalar words, the neighbourhood of the column it is in: W holds
+// the significance of the west, own and east columns for the six window
+// rows (bit 3k + j, window row k = stripe row k - 1, j = 0 west, 1 own,
+// 2 east) and S their signs in the same places.  Row ci's neighbourhood is
+// then (W >> 3ci) & 0x1ff (the ZC table index, NW N NE W C E SW S SE), so
+// the contexts are looked up when a row is coded, from words that already
+// hold this pass's decisions: a row that becomes significant sets one bit
+// in each word, and the next column's words take the west and own columns
+// of the current ones (W >> 1) when it follows a column with new
+// significance.  Nothing else is updated on the significant path.
+constexpr uint32_t kWinWestOwn = 0x1B6DBu;  // bits 3k and 3k + 1, k = 0..5
+
+// ZC context of row ci from the window word (the row's neighbourhood in
+// bits 3ci..3ci+8).
+__device__ __forceinline__ uint32_t win_zc(const Tables& t, uint32_t nb) {
+    return (rl(t.zc, (nb >> 3) & 63u) >> ((nb & 7u) * 4u)) & 15u;
+}
+// Sign decision of a row that became significant (decode/t1.rs sign coding
+// with the context of tier1/helpers.rs:53-89): nb and sb are the window
+// words shifted to the row; returns the sign.
+__device__ __forceinline__ uint32_t win_sign(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t nb,
+                                             uint32_t sb) {
+    const uint32_t si = (nb & 0xAAu) | ((sb & 0xAAu) >> 1);
     const uint32_t e = sc2_entry(sc2, si);
     return cd.code(t, e & 15u, 0) ^ (e >> 4);
 }
-
-// Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).
-// Lane-parallel at the start of each stripe, every column gets one word:
-// the ZC contexts of its four rows from the state at that point (nibbles
-// 0..3), the rows that can be coded (insignificant with a significant
-// neighbour, bits 16..19) and the insignificant rows (bits 20..23).  The
-// scalar scan then reads one word per column and one nibble per row; the
-// state it changes (rows that become significant, their column's and the
-// next column's words) is handled on the path of a significant decision
-// alone.  The scan visits the rows of the word, the row below one that
-// becomes significant, and, in the column after one with new significance,
-// the rows next to it.
-__device__ __forceinline__ uint32_t ctx4_lane(const Tables& t, uint32_t X) {
-    return zc_lane(t, X & 0x1ffu) | (zc_lane(t, (X >> 3) & 0x1ffu) << 4) | (zc_lane(t, (X >> 6) & 0x1ffu) << 8) |
-           (zc_lane(t, (X >> 9) & 0x1ffu) << 12);
-}
-__device__ __forceinline__ uint32_t ctx4_scalar(const Tables& t, uint32_t Xc) {
-    return zc_ctx(t, Xc & 0x1ffu) | (zc_ctx(t, (Xc >> 3) & 0x1ffu) << 4) | (zc_ctx(t, (Xc >> 6) & 0x1ffu) << 8) |
-           (zc_ctx(t, (Xc >> 9) & 0x1ffu) << 12);
-}
-// Scalar state of one stripe's scan.  fx: the column whose west neighbour
-// gained significance (the last column with a significant decision, plus
-// one); acc: the column that Xc, Yc (neighbourhood words with this pass's
-// changes), newC, newCs (rows that became significant and their signs), nX,
-// nY (their bits in the next column's words) and vadd (rows visited beyond
-// the word's) belong to.
-struct Scan {
-    uint32_t fx, acc, Xc, Yc, nX, nY, newC, newCs, vadd;
-};
-// The scan state starts from SGPR constants written by one asm: constants
+// Scan registers start from SGPR constants written by one asm: constants
 // the compiler materialises with v_mov could reach the MQ asm operands
 // through the loop phis as VGPRs.
-__device__ __forceinline__ Scan scan_init() {
+struct WinScan {
+    uint32_t W, S, fx, pnew;
+};
+__device__ __forceinline__ WinScan win_init() {
     uint32_t z, none;
     asm volatile("s_mov_b32 %0, 0\n\ts_mov_b32 %1, -1" : "=s"(z), "=s"(none));
-    return Scan{none, none, z, z, z, z, z, z, z};
+    return WinScan{z, z, none, z};
 }
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-// A value the compiler cannot see through (an empty asm), so that work on
-// the path of a significant decision is not hoisted to every column.
-__device__ __forceinline__ uint32_t opaque(uint32_t v) {
-    asm volatile("" : "+s"(v));
-    return v;
+// Enter column x: its words from the stripe-start words, plus the west and
+// own columns of the previous words when the previous column (x - 1) had
+// new significance.  Returns whether it had.
+__device__ __forceinline__ bool win_enter(WinScan& z, uint32_t X, uint32_t Y, uint32_t x) {
+    const bool adj = z.fx == x;
+    const uint32_t wa = (z.W >> 1) & kWinWestOwn, sa = (z.S >> 1) & kWinWestOwn;
+    z.W = uni(rl(X, x) | (adj ? wa : 0u));
+    z.S = uni(rl(Y, x) | (adj ? sa : 0u));
+    return adj;
 }
-// Start the accumulators of column x: its words, with the west column's new
-// significance when that column was the previous one with a significant
-// decision.
-__device__ __forceinline__ void scan_column(Scan& z, uint32_t X, uint32_t Y, uint32_t x) {
-    const bool west = z.fx == x;
-    z.Xc = uni(rl(X, x) | (west ? z.nX : 0u));
-    z.Yc = uni(rl(Y, x) | (west ? z.nY : 0u));
-    z.acc = x;
-    z.newC = z.newCs = z.nX = z.nY = z.vadd = uni(0u);
-}
-// A row of column x became significant: its sign, the words, and the
-// context of the row below (its north neighbour is now significant).
-__device__ __forceinline__ void scan_significant(Scan& z, Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t ci,
-                                                 uint32_t x, uint32_t& P) {
-    const uint32_t nb = (z.Xc >> (3u * ci)) & 0x1ffu;
-    const uint32_t sg = dec_sign(cd, t, sc2, nb, z.Yc, ci);
-    z.Xc |= 1u << (3u * ci + 4u);
-    z.Yc |= sg << (3u * ci + 4u);
-    z.nX |= 1u << (3u * ci + 3u);
-    z.nY |= sg << (3u * ci + 3u);
-    z.newC |= 1u << ci;
-    z.newCs |= sg << ci;
-    if (ci < 3u) {
-        const uint32_t c1 = ci + 1u;
-        P = (P & ~(15u << (4u * c1))) | (zc_ctx(t, (z.Xc >> (3u * c1)) & 0x1ffu) << (4u * c1));
-    }
-    z.fx = x + 1u;
-}
+
+// Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).
+// Coded: insignificant rows with a significant neighbour, in stripe-column
+// order.  Lane-parallel at the start of each stripe, every column gets one
+// word: those rows (bits 0..3) and the other insignificant rows (bits
+// 4..7), which become coded when a neighbour becomes significant during
+// the scan: the row below a new significance, and, in the next column, the
+// rows next to it.
 __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
                             int w, int h) {
     const int lane = lid();
     const int nstripes = (h + 3) >> 2;
+    const uint64_t cols = ballot(lane < w);
     for (int s = 0; s < nstripes; s++) {
         const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = (1u << nrows) - 1u;
@@ -1716,42 +1693,41 @@ __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         const uint32_t o6 = from_west(sig6) | sig6 | from_east(sig6);
         const uint32_t insig4 = (~(sig6 >> 1)) & rmask;
         const uint32_t cand = insig4 & (o6 | (o6 >> 1) | (o6 >> 2));
-        const uint32_t PL = ctx4_lane(t, X) | (cand << 16) | (insig4 << 20);
+        const uint32_t PL = cand | ((insig4 & ~cand) << 4);
         uint64_t M = ballot(lane < w && cand != 0);
         uint32_t upd = 0;
-        Scan z = scan_init();
+        WinScan z = win_init();
         while (M) {
             const uint32_t x = (uint32_t)__builtin_ctzll(M);
             M &= M - 1ull;
             cd.column();
-            uint32_t P = rl(PL, x);
-            uint32_t visit = (P >> 16) & 15u;
-            if (__builtin_expect(z.fx == x, 0)) {
-                // the west column gained significance: its bits, the
-                // contexts, and the rows that now have a significant neighbour
-                const uint32_t newW = z.newC;
-                scan_column(z, X, Y, x);
-                P = (P & ~0xffffu) | ctx4_scalar(t, z.Xc);
-                z.vadd = (P >> 20) & (newW | (newW << 1) | (newW >> 1)) & ~visit & 15u;
-                visit |= z.vadd;
-                if (z.vadd) upd = wl(upd, z.vadd << 8, opaque(x));
-                z.fx = 0xffffffffu;
-            }
+            const uint32_t P = rl(PL, x);
+            const uint32_t ix = P >> 4;
+            const uint32_t nw = win_enter(z, X, Y, x) ? z.pnew : 0u;
+            // rows next to the west column's new significance
+            const uint32_t vadd = (nw | (nw << 1) | (nw >> 1)) & ix;
+            uint32_t visit = (P & 15u) | vadd;
+            // bits 0..3: new significance, 4..7: its signs, 8..11: rows coded
+            // beyond the word's
+            uint32_t acc = vadd << 8;
             while (visit) {
                 const uint32_t ci = (uint32_t)__builtin_ctz(visit);
                 visit &= visit - 1u;
-                if (__builtin_expect(cd.code(t, (P >> (4u * ci)) & 15u, 0), 0)) {
-                    if (z.acc != x) scan_column(z, X, Y, x);
-                    scan_significant(z, cd, t, sc2, ci, x, P);
-                    // the row below now has a significant neighbour
-                    const uint32_t below = (P >> 20) & (2u << ci) & ~(P >> 16);
-                    visit |= (P >> 20) & (2u << ci);
-                    z.vadd |= below;
-                    const uint32_t xs = opaque(x);
-                    upd = wl(upd, z.newC | (z.newCs << 4) | (z.vadd << 8), xs);
-                    if (xs + 1u < (uint32_t)w) M |= 1ull << (xs + 1u);
+                const uint32_t sh = ci * 3u;
+                const uint32_t nb = z.W >> sh;
+                if (__builtin_expect(cd.code(t, win_zc(t, nb), 0), 0)) {
+                    const uint32_t sg = win_sign(cd, t, sc2, nb, z.S >> sh);
+                    z.W |= 1u << (sh + 4u);
+                    z.S |= sg << (sh + 4u);
+                    const uint32_t below = (2u << ci) & ix;  // its south neighbour
+                    visit |= below;
+                    acc |= (1u << ci) | (sg << (ci + 4u)) | (below << 8);
+                    M |= (2ull << x) & cols;
+                    z.fx = x + 1u;
                 }
             }
+            upd = wlane(upd, acc, x);
+            z.pnew = acc & 15u;
         }
         st.sig |= (uint64_t)(upd & 15u) << (4 * s);
         st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
@@ -1762,10 +1738,10 @@ __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
 
 // Cleanup pass with run-length mode on full clean stripes (t1.rs:977-1197,
 // decode/t1.rs:317-446).  Every row neither significant nor visited is
-// coded.  The column word holds the row contexts (nibbles 0..3), those rows
-// (bits 16..19) and whether the column starts in run mode (bit 20: four
-// rows, nothing significant around them, nothing visited); a column after
-// one with new significance loses run mode.
+// coded.  The column word holds those rows (bits 0..3) and whether the
+// column starts in run mode (bit 4: four rows, nothing significant around
+// them, nothing visited); a column after one with new significance loses
+// run mode.
 __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint32_t spr, Col& st, uint64_t& plane,
                             int w, int h) {
     const int lane = lid();
@@ -1779,45 +1755,49 @@ __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         // static part of stripe_is_clean (flags.rs:202-215)
         const uint32_t sclean = (nrows == 4 && X == 0 && vis4 == 0) ? 1u : 0u;
         const uint32_t todo = ~((sig6 >> 1) | vis4) & rmask;
-        const uint32_t PL = ctx4_lane(t, X) | (todo << 16) | (sclean << 20);
+        const uint32_t PL = todo | (sclean << 4);
         uint64_t M = ballot(lane < w && todo != 0);
         uint32_t upd = 0;
-        Scan z = scan_init();
+        WinScan z = win_init();
         while (M) {
             const uint32_t x = (uint32_t)__builtin_ctzll(M);
             M &= M - 1ull;
             cd.column();
-            uint32_t P = rl(PL, x);
-            uint32_t visit = (P >> 16) & 15u;
-            if (__builtin_expect(z.fx == x, 0)) {
-                scan_column(z, X, Y, x);
-                P = (P & ~0x10ffffu) | ctx4_scalar(t, z.Xc);
-                z.fx = 0xffffffffu;
-            }
-            if (P & 0x100000u) {  // run mode
+            const uint32_t P = rl(PL, x);
+            uint32_t visit = P & 15u;
+            uint32_t acc = 0;
+            if (!win_enter(z, X, Y, x) && (P & 16u)) {  // run mode
                 if (__builtin_expect(!cd.code(t, CTX_AGG, 0), 1)) {
                     visit = 0;
                 } else {
                     const uint32_t hi = cd.code(t, CTX_UNI, 0);
                     const uint32_t lo = cd.code(t, CTX_UNI, 0);
-                    const uint32_t rlen = (hi << 1) | lo;
-                    scan_column(z, X, Y, x);
+                    const uint32_t ci = (hi << 1) | lo;
+                    const uint32_t sh = ci * 3u;
                     // a clean stripe has no significant neighbour: sign
                     // index 0 (its contexts are 0 as well)
-                    scan_significant(z, cd, t, sc2, rlen, x, P);
-                    visit &= ~((2u << rlen) - 1u);  // rows after rlen
-                    upd = wl(upd, z.newC | (z.newCs << 4), opaque(x));
+                    const uint32_t sg = win_sign(cd, t, sc2, z.W >> sh, z.S >> sh);
+                    z.W |= 1u << (sh + 4u);
+                    z.S |= sg << (sh + 4u);
+                    acc = (1u << ci) | (sg << (ci + 4u));
+                    visit &= ~((2u << ci) - 1u);  // rows after it
+                    z.fx = x + 1u;
                 }
             }
             while (visit) {
                 const uint32_t ci = (uint32_t)__builtin_ctz(visit);
                 visit &= visit - 1u;
-                if (__builtin_expect(cd.code(t, (P >> (4u * ci)) & 15u, 0), 0)) {
-                    if (z.acc != x) scan_column(z, X, Y, x);
-                    scan_significant(z, cd, t, sc2, ci, x, P);
-                    upd = wl(upd, z.newC | (z.newCs << 4), opaque(x));
+                const uint32_t sh = ci * 3u;
+                const uint32_t nb = z.W >> sh;
+                if (__builtin_expect(cd.code(t, win_zc(t, nb), 0), 0)) {
+                    const uint32_t sg = win_sign(cd, t, sc2, nb, z.S >> sh);
+                    z.W |= 1u << (sh + 4u);
+                    z.S |= sg << (sh + 4u);
+                    acc |= (1u << ci) | (sg << (ci + 4u));
+                    z.fx = x + 1u;
                 }
             }
+            upd = wlane(upd, acc, x);
         }
         st.sig |= (uint64_t)(upd & 15u) << (4 * s);
         st.sgn |= (uint64_t)((upd >> 4) & 15u) << (4 * s);
