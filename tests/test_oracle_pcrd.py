@@ -185,7 +185,18 @@ CALIB_CASES = [("doc", lambda q: synth.document_page(200, 160, seed=q)),
                ("gradient", lambda q: synth.gradient_page(200, 160, seed=q)),
                ("flat", lambda q: np.clip(128 + np.random.default_rng(q).normal(0, 1.5, (160, 200)), 0, 255)
                 .astype(np.uint8))]
-_calibration_active = []
+def _max_hull_slope(enc):
+    """Steepest finite positive hull slope over the component's blocks,
+    recomputed from the raw curves."""
+    slopes = []
+    for i in range(enc.nblocks):
+        rb, rd, hp = enc.rd(i)
+        for a, b in zip(hp, hp[1:]):
+            db, dd = float(rb[b]) - float(rb[a]), rd[b] - rd[a]
+            s = math.inf if db == 0 and dd > 0 else (0.0 if db == 0 else dd / db)
+            if math.isfinite(s) and s > 0:
+                slopes.append(s)
+    return max(slopes)
 
 
 @pytest.mark.parametrize("kind,make", CALIB_CASES, ids=[c[0] for c in CALIB_CASES])
@@ -197,28 +208,24 @@ def test_lambda_calibration_closed_form(ora, q, kind, make):
     px = make(q)
     data, enc = ora.encode(px, q, keep=True)
     pixels = 200 * 160
-    slopes = []
-    for i in range(enc.nblocks):
-        rb, rd, hp = enc.rd(i)
-        for a, b in zip(hp, hp[1:]):
-            db, dd = float(rb[b]) - float(rb[a]), rd[b] - rd[a]
-            s = math.inf if db == 0 and dd > 0 else (0.0 if db == 0 else dd / db)
-            if math.isfinite(s) and s > 0:
-                slopes.append(s)
-    mx = max(slopes)
+    mx = _max_hull_slope(enc)
     assert enc.max_slope(0) == pytest.approx(mx, rel=1e-12)
     raw = ora.L.ora_quality_to_lambda(q, pixels)
     floor = ora.L.ora_quality_to_lambda(1, pixels)
     want = raw * min(1.0, 1.2 * mx / floor)
     assert enc.lam(0) == pytest.approx(want, rel=1e-12)
-    if 1.2 * mx < floor:
-        _calibration_active.append((kind, q))
 
 
-def test_lambda_calibration_was_exercised():
-    """The cases above include images whose steepest slope sits below
-    lambda(q=1), where the scaling is active."""
-    assert len(_calibration_active) >= 4, _calibration_active
+def test_lambda_calibration_was_exercised(ora):
+    """The cases of the closed-form test include images whose steepest
+    slope sits below lambda(q=1), where the scaling is active."""
+    active = []
+    for kind, make in CALIB_CASES:
+        for q in (20, 60, 85, 95):
+            _, enc = ora.encode(make(q), q, keep=True)
+            if 1.2 * _max_hull_slope(enc) < ora.L.ora_quality_to_lambda(1, 200 * 160):
+                active.append((kind, q))
+    assert len(active) >= 4, active
 
 
 # ---- brute-force Lagrangian selection -----------------------------------------------
