@@ -171,19 +171,12 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_cselect_b32 %[k], -1, 0\n\t"
         "s_mov_b32 %[a], %[q]\n\t"
         "s_branch 8f\n"
-        "2:\n\t"                        // shift with byte-in whenever ct is 0
-        "s_cmp_eq_u32 %[ct], 0\n\t"
-        "s_cbranch_scc1 4f\n"
-        "3:\n\t"
-        "s_min_u32 %[k], %[t], %[ct]\n\t"
-        "s_lshl_b32 %[a], %[a], %[k]\n\t"
-        "s_lshl_b32 %[c], %[c], %[k]\n\t"
-        "s_sub_u32 %[ct], %[ct], %[k]\n\t"
-        "s_sub_u32 %[t], %[t], %[k]\n\t"
-        "s_cmp_eq_u32 %[t], 0\n\t"
-        "s_cbranch_scc0 2b\n\t"
-        "s_branch 9f\n"
-        "4:\n\t"                    // mq/mod.rs:133-155
+        "2:\n\t"                    // t > ct: shift by ct, then a byte-in
+        "s_lshl_b32 %[a], %[a], %[ct]\n\t"
+        "s_lshl_b32 %[c], %[c], %[ct]\n\t"
+        "s_sub_u32 %[t], %[t], %[ct]\n\t"
+        // byte-in, ct is 0 (mq/mod.rs:133-155): the usual byte (the held
+        // one not 0xFF) runs straight through
         "s_sub_u32 %[k], %[bp], %[db]\n\t"
         "s_add_u32 %[k], %[k], 1\n\t"
         "v_readlane_b32 %[n0], %[w0], %[k]\n\t"
@@ -191,13 +184,22 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_cmp_lt_u32 %[k], 64\n\t"
         "s_cselect_b32 %[n0], %[n0], %[n1]\n\t"
         "s_cmp_eq_u32 %[cur], 0xff\n\t"
-        "s_cbranch_scc0 6f\n\t"
+        "s_cbranch_scc1 5f\n\t"
+        "s_add_u32 %[bp], %[bp], 1\n\t"
+        "s_mov_b32 %[cur], %[n0]\n\t"
+        "s_lshl_b32 %[n0], %[n0], 8\n\t"
+        "s_add_u32 %[c], %[c], %[n0]\n\t"
+        "s_mov_b32 %[ct], 8\n"
+        "3:\n\t"                    // the rest of the shift
+        "s_cmp_gt_u32 %[t], %[ct]\n\t"
+        "s_cbranch_scc1 2b\n\t"
+        "s_lshl_b32 %[a], %[a], %[t]\n\t"
+        "s_lshl_b32 %[c], %[c], %[t]\n\t"
+        "s_sub_u32 %[ct], %[ct], %[t]\n\t"
+        "s_branch 9f\n"
+        "5:\n\t"                    // after 0xFF: a 7-bit byte, or a marker
         "s_cmp_gt_u32 %[n0], 0x8f\n\t"
-        "s_cbranch_scc0 5f\n\t"
-        "s_add_u32 %[c], %[c], 0xff00\n\t"
-        "s_mov_b32 %[ct], 8\n\t"
-        "s_branch 3b\n"
-        "5:\n\t"
+        "s_cbranch_scc1 6f\n\t"
         "s_add_u32 %[bp], %[bp], 1\n\t"
         "s_mov_b32 %[cur], %[n0]\n\t"
         "s_lshl_b32 %[n0], %[n0], 9\n\t"
@@ -205,10 +207,7 @@ __device__ __forceinline__ uint32_t mq_decode(MqDec& d, const Tables&, uint32_t 
         "s_mov_b32 %[ct], 7\n\t"
         "s_branch 3b\n"
         "6:\n\t"
-        "s_add_u32 %[bp], %[bp], 1\n\t"
-        "s_mov_b32 %[cur], %[n0]\n\t"
-        "s_lshl_b32 %[n0], %[n0], 8\n\t"
-        "s_add_u32 %[c], %[c], %[n0]\n\t"
+        "s_add_u32 %[c], %[c], 0xff00\n\t"
         "s_mov_b32 %[ct], 8\n\t"
         "s_branch 3b\n"
         "7:\n\t"                        // upper sub-interval, a < 0x8000: LPS iff a < qe
