@@ -1653,23 +1653,40 @@ __device__ __forceinline__ uint32_t win_sign(Coder<true>& cd, const Tables& t, u
 // the compiler materialises with v_mov could reach the MQ asm operands
 // through the loop phis as VGPRs.
 struct WinScan {
-    uint32_t W, S, fx, pnew;
+    uint32_t W, Sx, fx, pnew;
 };
 __device__ __forceinline__ WinScan win_init() {
     uint32_t z, none;
     asm volatile("s_mov_b32 %0, 0\n\ts_mov_b32 %1, -1" : "=s"(z), "=s"(none));
     return WinScan{z, z, none, z};
 }
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-// Enter column x: its words from the stripe-start words, plus the west and
-// own columns of the previous words when the previous column (x - 1) had
-// new significance.  Returns whether it had.
-__device__ __forceinline__ bool win_enter(WinScan& z, uint32_t X, uint32_t Y, uint32_t x) {
-    const bool adj = z.fx == x;
-    const uint32_t wa = (z.W >> 1) & kWinWestOwn, sa = (z.S >> 1) & kWinWestOwn;
-    z.W = uni(rl(X, x) | (adj ? wa : 0u));
-    z.S = uni(rl(Y, x) | (adj ? sa : 0u));
-    return adj;
+// Column x leaves the 64-bit candidate mask (one instruction).
+__device__ __forceinline__ void drop_column(uint64_t& m, uint32_t x) {
+    asm volatile("s_bitset0_b64 %0, %1" : "+s"(m) : "s"(x));
+}
+// The column after x joins the scan (if the block has one).  cols is the
+// block's column mask without column 0: s_bitset1_b64 takes the bit index
+// mod 64, so for x = 63 it sets bit 0, which the mask then clears.
+__device__ __forceinline__ void next_column(uint64_t& m, uint32_t x, uint64_t cols) {
+    asm volatile("s_bitset1_b64 %0, %1\n\ts_and_b64 %0, %0, %2" : "+s"(m) : "s"(x + 1u), "s"(cols) : "scc");
+}
+// Column x's words: the significance word from the stripe-start words X,
+// plus, when the previous column (x - 1) had new significance, the west
+// and own columns of that column's word shifted into place (a real branch:
+// the empty asm keeps the compiler from turning it into selects on every
+// column).  The column's signs are the stripe-start word Y plus Sx (those
+// merged bits and the signs decoded in the column); Y is read only when a
+// row turns significant.
+// Row ci (constant after unrolling) of column x turned significant: decode
+// its sign and record both in the column words.
+__device__ __forceinline__ uint32_t win_newsig(Coder<true>& cd, const Tables& t, uint32_t sc2, WinScan& z, uint32_t Y,
+                                               uint32_t x, uint32_t ci) {
+    const uint32_t sh = ci * 3u;
+    const uint32_t sg = win_sign(cd, t, sc2, z.W >> sh, (rl(Y, x) | z.Sx) >> sh);
+    z.W |= 1u << (sh + 4u);
+    z.Sx |= sg << (sh + 4u);
+    z.fx = x + 1u;
+    return sg;
 }
 
 // Significance propagation pass (t1.rs:805-905, decode/t1.rs:211-266).
@@ -1683,7 +1700,7 @@ __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
                             int w, int h) {
     const int lane = lid();
     const int nstripes = (h + 3) >> 2;
-    const uint64_t cols = ballot(lane < w);
+    const uint64_t cols = ballot(lane < w) & ~1ull;  // the columns another column can add
     for (int s = 0; s < nstripes; s++) {
         const int nrows = __builtin_amdgcn_readfirstlane(min(4, h - 4 * s));
         const uint32_t rmask = (1u << nrows) - 1u;
@@ -1698,31 +1715,36 @@ __device__ void sp_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         WinScan z = win_init();
         while (M) {
             const uint32_t x = (uint32_t)__builtin_ctzll(M);
-            M &= M - 1ull;
+            drop_column(M, x);
             cd.column();
             const uint32_t P = rl(PL, x);
             const uint32_t ix = P >> 4;
-            const uint32_t nw = win_enter(z, X, Y, x) ? z.pnew : 0u;
-            // rows next to the west column's new significance
-            const uint32_t vadd = (nw | (nw << 1) | (nw >> 1)) & ix;
-            uint32_t visit = (P & 15u) | vadd;
-            // bits 0..3: new significance, 4..7: its signs, 8..11: rows coded
-            // beyond the word's
-            uint32_t acc = vadd << 8;
-            while (visit) {
-                const uint32_t ci = (uint32_t)__builtin_ctz(visit);
-                visit &= visit - 1u;
-                const uint32_t sh = ci * 3u;
-                const uint32_t nb = z.W >> sh;
-                if (__builtin_expect(cd.code(t, win_zc(t, nb), 0), 0)) {
-                    const uint32_t sg = win_sign(cd, t, sc2, nb, z.S >> sh);
-                    z.W |= 1u << (sh + 4u);
-                    z.S |= sg << (sh + 4u);
-                    const uint32_t below = (2u << ci) & ix;  // its south neighbour
-                    visit |= below;
-                    acc |= (1u << ci) | (sg << (ci + 4u)) | (below << 8);
-                    M |= (2ull << x) & cols;
-                    z.fx = x + 1u;
+            // acc bits 0..3: new significance, 4..7: its signs, 8..11: rows
+            // coded beyond the word's
+            const uint32_t Wp = z.W, Sp = z.Sx;
+            z.W = rl(X, x);
+            z.Sx = 0;
+            uint32_t visit = P & 15u, acc = 0;
+            if (__builtin_expect(z.fx == x, 0)) {
+                asm volatile("");
+                z.W |= (Wp >> 1) & kWinWestOwn;
+                z.Sx = ((rl(Y, x - 1u) | Sp) >> 1) & kWinWestOwn;
+                // rows next to the west column's new significance
+                const uint32_t nw = z.pnew;
+                const uint32_t vadd = (nw | (nw << 1) | (nw >> 1)) & ix;
+                visit |= vadd;
+                acc = vadd << 8;
+            }
+#pragma unroll
+            for (uint32_t ci = 0; ci < 4u; ci++) {
+                if (visit & (1u << ci)) {
+                    if (__builtin_expect(cd.code(t, win_zc(t, z.W >> (3u * ci)), 0), 0)) {
+                        const uint32_t sg = win_newsig(cd, t, sc2, z, Y, x, ci);
+                        const uint32_t below = (2u << ci) & ix;  // its south neighbour
+                        visit |= below;
+                        acc |= (1u << ci) | (sg << (ci + 4u)) | (below << 8);
+                        next_column(M, x, cols);
+                    }
                 }
             }
             upd = wlane(upd, acc, x);
@@ -1760,12 +1782,20 @@ __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
         WinScan z = win_init();
         while (M) {
             const uint32_t x = (uint32_t)__builtin_ctzll(M);
-            M &= M - 1ull;
+            drop_column(M, x);
             cd.column();
             const uint32_t P = rl(PL, x);
+            const uint32_t Wp = z.W, Sp = z.Sx;
+            z.W = rl(X, x);
+            z.Sx = 0;
             uint32_t visit = P & 15u;
             uint32_t acc = 0;
-            if (!win_enter(z, X, Y, x) && (P & 16u)) {  // run mode
+            if (__builtin_expect(z.fx == x, 0)) {
+                // the west column had new significance (no run mode then)
+                asm volatile("");
+                z.W |= (Wp >> 1) & kWinWestOwn;
+                z.Sx = ((rl(Y, x - 1u) | Sp) >> 1) & kWinWestOwn;
+            } else if (P & 16u) {  // run mode
                 if (__builtin_expect(!cd.code(t, CTX_AGG, 0), 1)) {
                     visit = 0;
                 } else {
@@ -1775,25 +1805,21 @@ __device__ void cl_pass_dec(Coder<true>& cd, const Tables& t, uint32_t sc2, uint
                     const uint32_t sh = ci * 3u;
                     // a clean stripe has no significant neighbour: sign
                     // index 0 (its contexts are 0 as well)
-                    const uint32_t sg = win_sign(cd, t, sc2, z.W >> sh, z.S >> sh);
+                    const uint32_t sg = win_sign(cd, t, sc2, z.W >> sh, z.Sx >> sh);
                     z.W |= 1u << (sh + 4u);
-                    z.S |= sg << (sh + 4u);
+                    z.Sx |= sg << (sh + 4u);
                     acc = (1u << ci) | (sg << (ci + 4u));
                     visit &= ~((2u << ci) - 1u);  // rows after it
                     z.fx = x + 1u;
                 }
             }
-            while (visit) {
-                const uint32_t ci = (uint32_t)__builtin_ctz(visit);
-                visit &= visit - 1u;
-                const uint32_t sh = ci * 3u;
-                const uint32_t nb = z.W >> sh;
-                if (__builtin_expect(cd.code(t, win_zc(t, nb), 0), 0)) {
-                    const uint32_t sg = win_sign(cd, t, sc2, nb, z.S >> sh);
-                    z.W |= 1u << (sh + 4u);
-                    z.S |= sg << (sh + 4u);
-                    acc |= (1u << ci) | (sg << (ci + 4u));
-                    z.fx = x + 1u;
+#pragma unroll
+            for (uint32_t ci = 0; ci < 4u; ci++) {
+                if (visit & (1u << ci)) {
+                    if (__builtin_expect(cd.code(t, win_zc(t, z.W >> (3u * ci)), 0), 0)) {
+                        const uint32_t sg = win_newsig(cd, t, sc2, z, Y, x, ci);
+                        acc |= (1u << ci) | (sg << (ci + 4u));
+                    }
                 }
             }
             upd = wlane(upd, acc, x);
