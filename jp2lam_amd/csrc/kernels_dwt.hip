@@ -3,9 +3,13 @@
 //
 // Each DWT level is one launch of 2-D tile kernels: a workgroup loads a
 // 64x64 output tile plus a 4-sample mirrored halo into LDS, runs the
-// vertical then horizontal lifting passes there (inverse: horizontal then
+// vertical then horizontal lifting passes (inverse: horizontal then
 // vertical) and writes the four subbands once.  Every level therefore reads
-// and writes its active region exactly once from HBM.
+// and writes its active region once from HBM (plus the halo).  In each pass
+// a thread lifts a window of 24 samples of one column (row) in registers:
+// 16 outputs with 4 samples either side, all lifting steps and the scaling
+// at once, so a pass costs one LDS read and one write per sample and one
+// barrier, not one per lifting step.
 //
 // Arithmetic follows the reference's lifting order exactly
 // (dwt/irrev97.rs:92-129, :261-336, :625-820; dwt/rev53.rs:143-417):
@@ -25,6 +29,12 @@ constexpr int TX = 64, TY = 64, HALO = 4;
 constexpr int TLW = TX + 2 * HALO;      // 72
 constexpr int TLH = TY + 2 * HALO;      // 72
 constexpr int TLP = TLW + 1;             // padded row pitch
+// A pass splits the 64 outputs of a column (row) into 4 segments of 16; a
+// segment's window is 16 + 2 * HALO samples.  288 threads: 72 columns x 4
+// segments in the forward vertical and the inverse horizontal pass.
+constexpr int SEG = 16, NSEG = TY / SEG, WIN = SEG + 2 * HALO;
+constexpr int kLevelThreads = TLW * NSEG;  // 288
+constexpr int kLoadsPerThread = TLH / NSEG;  // the tile load: 18 rows per thread
 
 // irrev97.rs:10-15
 __constant__ float kAlpha = -1.586134342059924f;
@@ -47,80 +57,62 @@ __device__ __forceinline__ float lift(float x, float c, float l, float r) {
     return __fadd_rn(x, __fmul_rn(c, __fadd_rn(l, r)));
 }
 
-// One lifting step over rows of one parity (vertical pass) for all columns
-// in [c0, c1).  Rows in [1, nrows-2] of the given parity are updated.
-template <bool IRREV, typename T>
-__device__ __forceinline__ void vstep(T (*s)[TLP], int par, int step, int c0, int c1, int nrows) {
-    const int ncols = c1 - c0;
-    const int first = par ? 1 : 2;
-    const int nr = (nrows - 1 - first + 1) / 2;  // rows first, first+2, ... <= nrows-2
-    for (int idx = threadIdx.x; idx < nr * ncols; idx += blockDim.x) {
-        int j = first + 2 * (idx / ncols);
-        int c = c0 + idx % ncols;
-        if constexpr (IRREV) {
-            float cc = step == 0 ? kAlpha : step == 1 ? kBeta : step == 2 ? kGamma : kDelta;
-            s[j][c] = lift(s[j][c], cc, s[j - 1][c], s[j + 1][c]);
-        } else {
-            if (step == 0) s[j][c] -= (s[j - 1][c] + s[j + 1][c]) >> 1;
-            else s[j][c] += (s[j - 1][c] + s[j + 1][c] + 2) >> 2;
-        }
+// One lifting step on the window's samples of parity `par` in [1, WIN-2]
+// (window position parity is global parity: windows start at even tile
+// rows/columns and tiles at multiples of 64).  The window's outer samples
+// take stale values as the steps go on, one more per step from each end,
+// so after the four 9/7 steps (two 5/3 steps) positions [HALO, HALO + SEG)
+// equal the lifting of the whole signal (irrev97.rs:158-228, :339-399;
+// rev53.rs:207-265).
+template <int PAR>
+__device__ __forceinline__ void lift_step(float (&v)[WIN], float c) {
+#pragma unroll
+    for (int j = PAR ? 1 : 2; j <= WIN - 2; j += 2) v[j] = lift(v[j], c, v[j - 1], v[j + 1]);
+}
+// 5/3 predict on odd samples (x -= (l + r) >> 1) and update on even ones
+// (x += (l + r + 2) >> 2); the inverse undoes them in the other order.
+__device__ __forceinline__ void predict53(int32_t (&v)[WIN], int sign) {
+#pragma unroll
+    for (int j = 1; j <= WIN - 2; j += 2) {
+        const int32_t d = (v[j - 1] + v[j + 1]) >> 1;
+        v[j] = sign > 0 ? v[j] - d : v[j] + d;
     }
 }
-
-template <bool IRREV, typename T>
-__device__ __forceinline__ void hstep(T (*s)[TLP], int par, int step, int r0, int r1, int ncols) {
-    const int nrows = r1 - r0;
-    const int first = par ? 1 : 2;
-    const int nc = (ncols - 1 - first + 1) / 2;
-    for (int idx = threadIdx.x; idx < nc * nrows; idx += blockDim.x) {
-        int r = r0 + idx / nc;
-        int j = first + 2 * (idx % nc);
-        if constexpr (IRREV) {
-            float cc = step == 0 ? kAlpha : step == 1 ? kBeta : step == 2 ? kGamma : kDelta;
-            s[r][j] = lift(s[r][j], cc, s[r][j - 1], s[r][j + 1]);
-        } else {
-            if (step == 0) s[r][j] -= (s[r][j - 1] + s[r][j + 1]) >> 1;
-            else s[r][j] += (s[r][j - 1] + s[r][j + 1] + 2) >> 2;
-        }
+__device__ __forceinline__ void update53(int32_t (&v)[WIN], int sign) {
+#pragma unroll
+    for (int j = 2; j <= WIN - 2; j += 2) {
+        const int32_t d = (v[j - 1] + v[j + 1] + 2) >> 2;
+        v[j] = sign > 0 ? v[j] + d : v[j] - d;
     }
 }
-
-// Inverse steps: 9/7 uses -delta (even), -gamma (odd), -beta (even),
-// -alpha (odd); 5/3 undoes update (even) then predict (odd).
-template <bool IRREV, typename T>
-__device__ __forceinline__ void vstep_inv(T (*s)[TLP], int par, int step, int c0, int c1, int nrows) {
-    const int ncols = c1 - c0;
-    const int first = par ? 1 : 2;
-    const int nr = (nrows - 1 - first + 1) / 2;
-    for (int idx = threadIdx.x; idx < nr * ncols; idx += blockDim.x) {
-        int j = first + 2 * (idx / ncols);
-        int c = c0 + idx % ncols;
-        if constexpr (IRREV) {
-            float cc = step == 0 ? -kDelta : step == 1 ? -kGamma : step == 2 ? -kBeta : -kAlpha;
-            s[j][c] = lift(s[j][c], cc, s[j - 1][c], s[j + 1][c]);
-        } else {
-            if (step == 0) s[j][c] -= (s[j - 1][c] + s[j + 1][c] + 2) >> 2;
-            else s[j][c] += (s[j - 1][c] + s[j + 1][c]) >> 1;
-        }
-    }
+// Forward 1-D transform of a window: the four lifts, then even x 1/K, odd
+// x K (9/7); predict then update (5/3).
+__device__ __forceinline__ void fwd_window(float (&v)[WIN]) {
+    lift_step<1>(v, kAlpha);
+    lift_step<0>(v, kBeta);
+    lift_step<1>(v, kGamma);
+    lift_step<0>(v, kDelta);
+#pragma unroll
+    for (int j = 0; j < WIN; j++) v[j] = __fmul_rn(v[j], (j & 1) ? K97 : INV_K97);
 }
-
-template <bool IRREV, typename T>
-__device__ __forceinline__ void hstep_inv(T (*s)[TLP], int par, int step, int r0, int r1, int ncols) {
-    const int nrows = r1 - r0;
-    const int first = par ? 1 : 2;
-    const int nc = (ncols - 1 - first + 1) / 2;
-    for (int idx = threadIdx.x; idx < nc * nrows; idx += blockDim.x) {
-        int r = r0 + idx / nc;
-        int j = first + 2 * (idx % nc);
-        if constexpr (IRREV) {
-            float cc = step == 0 ? -kDelta : step == 1 ? -kGamma : step == 2 ? -kBeta : -kAlpha;
-            s[r][j] = lift(s[r][j], cc, s[r][j - 1], s[r][j + 1]);
-        } else {
-            if (step == 0) s[r][j] -= (s[r][j - 1] + s[r][j + 1] + 2) >> 2;
-            else s[r][j] += (s[r][j - 1] + s[r][j + 1]) >> 1;
-        }
-    }
+__device__ __forceinline__ void fwd_window(int32_t (&v)[WIN]) {
+    predict53(v, 1);
+    update53(v, 1);
+}
+// Inverse 1-D transform (irrev97.rs:636-820; rev53.rs:272-417): even x K,
+// odd x 1/K, then -delta (even), -gamma (odd), -beta (even), -alpha (odd);
+// 5/3: undo the update, then the predict.
+__device__ __forceinline__ void inv_window(float (&v)[WIN]) {
+#pragma unroll
+    for (int j = 0; j < WIN; j++) v[j] = __fmul_rn(v[j], (j & 1) ? INV_K97 : K97);
+    lift_step<0>(v, -kDelta);
+    lift_step<1>(v, -kGamma);
+    lift_step<0>(v, -kBeta);
+    lift_step<1>(v, -kAlpha);
+}
+__device__ __forceinline__ void inv_window(int32_t (&v)[WIN]) {
+    update53(v, -1);
+    predict53(v, -1);
 }
 
 }  // namespace
@@ -144,64 +136,78 @@ __device__ __forceinline__ int find_seg(const DwtSeg* segs, int nseg, uint32_t b
 // Subbands: LL -> out, HL/TLH/HH -> plane at their Mallat offsets.  Tile
 // origins are multiples of 64 so tile-local parity equals global parity.
 template <typename T, bool IRREV>
-__global__ __launch_bounds__(1024) void fwd_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
+__global__ __launch_bounds__(kLevelThreads) void fwd_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
     __shared__ T s[TLH][TLP];
     const DwtSeg S = segs[find_seg(segs, nseg, blockIdx.x)];
     const uint32_t t = blockIdx.x - S.tile_start;
     const int tx0 = (int)(t % (uint32_t)S.tiles_x) * TX, ty0 = (int)(t / (uint32_t)S.tiles_x) * TY;
     const int rw = S.rw, rh = S.rh;
+    const int tid = threadIdx.x;
     const T* __restrict__ in = (const T*)S.in;
     T* __restrict__ plane = (T*)S.plane;
     T* __restrict__ ll_out = (T*)S.out;
-    for (int idx = threadIdx.x; idx < TLH * TLW; idx += blockDim.x) {
-        int ly = idx / TLW, lx = idx % TLW;
-        int gy = sym(rh, ty0 - HALO + ly), gx = sym(rw, tx0 - HALO + lx);
-        s[ly][lx] = in[(size_t)gy * S.in_stride + gx];
+    // tile load: thread = column lx, rows ly0, ly0 + 4, ...; all 18 loads in
+    // flight before the LDS writes
+    {
+        const int lx = tid % TLW, ly0 = tid / TLW;
+        const int gx = sym(rw, tx0 - HALO + lx);
+        T r[kLoadsPerThread];
+#pragma unroll
+        for (int k = 0; k < kLoadsPerThread; k++) {
+            const int gy = sym(rh, ty0 - HALO + ly0 + NSEG * k);
+            r[k] = in[(size_t)gy * S.in_stride + gx];
+        }
+#pragma unroll
+        for (int k = 0; k < kLoadsPerThread; k++) s[ly0 + NSEG * k][lx] = r[k];
     }
     __syncthreads();
-    // vertical pass (irrev97.rs:261-336; rev53.rs:419-497)
-    if (rh >= 2) {
-        if constexpr (IRREV) {
-            for (int st = 0; st < 4; st++) {
-                vstep<true>(s, (st & 1) ? 0 : 1, st, 0, TLW, TLH);
-                __syncthreads();
-            }
-            for (int idx = threadIdx.x; idx < TLH * TLW; idx += blockDim.x) {
-                int ly = idx / TLW, lx = idx % TLW;
-                s[ly][lx] = __fmul_rn(s[ly][lx], (ly & 1) ? K97 : INV_K97);
-            }
+    // vertical pass (irrev97.rs:261-336; rev53.rs:419-497): thread = column
+    // c of all 72, segment g; writes back the segment's 16 output rows
+    {
+        const int c = tid % TLW, g = tid / TLW;
+        if (rh >= 2) {
+            T v[WIN];
+#pragma unroll
+            for (int j = 0; j < WIN; j++) v[j] = s[g * SEG + j][c];
+            fwd_window(v);
+            __syncthreads();  // every window read before any write-back
+#pragma unroll
+            for (int j = 0; j < SEG; j++) s[g * SEG + HALO + j][c] = v[HALO + j];
         } else {
-            vstep<false>(s, 1, 0, 0, TLW, TLH);
             __syncthreads();
-            vstep<false>(s, 0, 1, 0, TLW, TLH);
+            if constexpr (IRREV) {
+                if (g == 0) s[HALO][c] = __fmul_rn(s[HALO][c], INV_K97);
+            }
         }
-    } else if constexpr (IRREV) {
-        for (int lx = threadIdx.x; lx < TLW; lx += blockDim.x) s[HALO][lx] = __fmul_rn(s[HALO][lx], INV_K97);
     }
     __syncthreads();
-    // horizontal pass on the output rows (irrev97.rs:92-129; rev53.rs:143-265)
-    if (rw >= 2) {
-        if constexpr (IRREV) {
-            for (int st = 0; st < 4; st++) {
-                hstep<true>(s, (st & 1) ? 0 : 1, st, HALO, HALO + TY, TLW);
-                __syncthreads();
-            }
-            for (int idx = threadIdx.x; idx < TY * TLW; idx += blockDim.x) {
-                int ly = HALO + idx / TLW, lx = idx % TLW;
-                s[ly][lx] = __fmul_rn(s[ly][lx], (lx & 1) ? K97 : INV_K97);
-            }
-        } else {
-            hstep<false>(s, 1, 0, HALO, HALO + TY, TLW);
+    // horizontal pass on the output rows (irrev97.rs:92-129; rev53.rs:143-265):
+    // thread = output row r, segment g (256 threads)
+    if (tid < TY * NSEG) {
+        const int r = HALO + (tid % TY), g = tid / TY;
+        if (rw >= 2) {
+            T v[WIN];
+#pragma unroll
+            for (int j = 0; j < WIN; j++) v[j] = s[r][g * SEG + j];
+            fwd_window(v);
+            // the windows of neighbouring segments overlap by 8 samples:
+            // every read before any write-back
             __syncthreads();
-            hstep<false>(s, 0, 1, HALO, HALO + TY, TLW);
+#pragma unroll
+            for (int j = 0; j < SEG; j++) s[r][g * SEG + HALO + j] = v[HALO + j];
+        } else {
+            __syncthreads();
+            if constexpr (IRREV) {
+                if (g == 0) s[r][HALO] = __fmul_rn(s[r][HALO], INV_K97);
+            }
         }
-    } else if constexpr (IRREV) {
-        for (int ly = threadIdx.x; ly < TLH; ly += blockDim.x) s[ly][HALO] = __fmul_rn(s[ly][HALO], INV_K97);
+    } else {
+        __syncthreads();
     }
     __syncthreads();
     // HL and HH start at plane column S.hx, LH and HH at Mallat row snh
     const int snh = (rh + 1) >> 1, bhx = (int)S.hx;
-    for (int idx = threadIdx.x; idx < TY * TX; idx += blockDim.x) {
+    for (int idx = tid; idx < TY * TX; idx += kLevelThreads) {
         int oy = idx / TX, ox = idx % TX;
         int gy = ty0 + oy, gx = tx0 + ox;
         if (gy >= rh || gx >= rw) continue;
@@ -220,76 +226,78 @@ __global__ __launch_bounds__(1024) void fwd_level_kernel(const DwtSeg* __restric
 // One inverse level: LL from `in`, HL/TLH/HH from `plane`, interleaved
 // rw x rh result into `out` (irrev97.rs:636-820; rev53.rs:95-136, :272-707).
 template <typename T, bool IRREV>
-__global__ __launch_bounds__(1024) void inv_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
+__global__ __launch_bounds__(kLevelThreads) void inv_level_kernel(const DwtSeg* __restrict__ segs, int nseg) {
     __shared__ T s[TLH][TLP];
     const DwtSeg S = segs[find_seg(segs, nseg, blockIdx.x)];
     const uint32_t t = blockIdx.x - S.tile_start;
     const int tx0 = (int)(t % (uint32_t)S.tiles_x) * TX, ty0 = (int)(t / (uint32_t)S.tiles_x) * TY;
     const int rw = S.rw, rh = S.rh;
+    const int tid = threadIdx.x;
     const T* __restrict__ ll_in = (const T*)S.in;
     const T* __restrict__ plane = (const T*)S.plane;
     T* __restrict__ out = (T*)S.out;
     const int snh = (rh + 1) >> 1, bhx = (int)S.hx;
-    for (int idx = threadIdx.x; idx < TLH * TLW; idx += blockDim.x) {
-        int ly = idx / TLW, lx = idx % TLW;
-        int my = sym(rh, ty0 - HALO + ly), mx = sym(rw, tx0 - HALO + lx);
-        int hy = my >> 1, hx = mx >> 1;
-        T v;
-        if (my & 1) v = plane[(size_t)(snh + hy) * S.pstride + ((mx & 1) ? bhx + hx : hx)];
-        else if (mx & 1) v = plane[(size_t)hy * S.pstride + bhx + hx];
-        else v = ll_in[(size_t)hy * S.in_stride + hx];
-        s[ly][lx] = v;
+    // tile gather from the four bands: thread = column lx, rows ly0, ly0 + 4,
+    // ...; all 18 loads in flight before the LDS writes
+    {
+        const int lx = tid % TLW, ly0 = tid / TLW;
+        const int mx = sym(rw, tx0 - HALO + lx), hx = mx >> 1;
+        const int hcol = (mx & 1) ? bhx + hx : hx;  // column in the plane
+        T r[kLoadsPerThread];
+#pragma unroll
+        for (int k = 0; k < kLoadsPerThread; k++) {
+            const int my = sym(rh, ty0 - HALO + ly0 + NSEG * k), hy = my >> 1;
+            const T* src = (my & 1) ? plane + (size_t)(snh + hy) * S.pstride + hcol
+                         : (mx & 1) ? plane + (size_t)hy * S.pstride + hcol
+                                    : ll_in + (size_t)hy * S.in_stride + hx;
+            r[k] = *src;
+        }
+#pragma unroll
+        for (int k = 0; k < kLoadsPerThread; k++) s[ly0 + NSEG * k][lx] = r[k];
     }
     __syncthreads();
-    // horizontal inverse on every tile row (the vertical pass needs them)
-    if (rw >= 2) {
-        if constexpr (IRREV) {
-            for (int idx = threadIdx.x; idx < TLH * TLW; idx += blockDim.x) {
-                int ly = idx / TLW, lx = idx % TLW;
-                s[ly][lx] = __fmul_rn(s[ly][lx], (lx & 1) ? INV_K97 : K97);
-            }
+    // horizontal inverse on every tile row (the vertical pass needs them):
+    // thread = row r of all 72, segment g; writes back the 16 output columns
+    {
+        const int r = tid % TLH, g = tid / TLH;
+        if (rw >= 2) {
+            T v[WIN];
+#pragma unroll
+            for (int j = 0; j < WIN; j++) v[j] = s[r][g * SEG + j];
+            inv_window(v);
             __syncthreads();
-            for (int st = 0; st < 4; st++) {
-                hstep_inv<true>(s, st & 1, st, 0, TLH, TLW);
-                __syncthreads();
-            }
+#pragma unroll
+            for (int j = 0; j < SEG; j++) s[r][g * SEG + HALO + j] = v[HALO + j];
         } else {
-            hstep_inv<false>(s, 0, 0, 0, TLH, TLW);
             __syncthreads();
-            hstep_inv<false>(s, 1, 1, 0, TLH, TLW);
-            __syncthreads();
+            if constexpr (IRREV) {
+                if (g == 0) s[r][HALO] = __fmul_rn(s[r][HALO], K97);
+            }
         }
-    } else if constexpr (IRREV) {
-        for (int ly = threadIdx.x; ly < TLH; ly += blockDim.x) s[ly][HALO] = __fmul_rn(s[ly][HALO], K97);
-        __syncthreads();
     }
-    // vertical inverse on the output columns
-    if (rh >= 2) {
-        if constexpr (IRREV) {
-            for (int idx = threadIdx.x; idx < TLH * TX; idx += blockDim.x) {
-                int ly = idx / TX, lx = HALO + idx % TX;
-                s[ly][lx] = __fmul_rn(s[ly][lx], (ly & 1) ? INV_K97 : K97);
-            }
-            __syncthreads();
-            for (int st = 0; st < 4; st++) {
-                vstep_inv<true>(s, st & 1, st, HALO, HALO + TX, TLH);
-                __syncthreads();
-            }
+    __syncthreads();
+    // vertical inverse on the output columns, straight to the output:
+    // thread = output column c, segment g (256 threads)
+    if (tid < TX * NSEG) {
+        const int c = HALO + (tid % TX), g = tid / TX;
+        const int gx = tx0 + c - HALO;
+        T v[WIN];
+        if (rh >= 2) {
+#pragma unroll
+            for (int j = 0; j < WIN; j++) v[j] = s[g * SEG + j][c];
+            inv_window(v);
         } else {
-            vstep_inv<false>(s, 0, 0, HALO, HALO + TX, TLH);
-            __syncthreads();
-            vstep_inv<false>(s, 1, 1, HALO, HALO + TX, TLH);
-            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < WIN; j++) v[j] = s[g * SEG + j][c];
+            if constexpr (IRREV) v[HALO] = __fmul_rn(v[HALO], K97);
         }
-    } else if constexpr (IRREV) {
-        for (int lx = threadIdx.x; lx < TLW; lx += blockDim.x) s[HALO][lx] = __fmul_rn(s[HALO][lx], K97);
-        __syncthreads();
-    }
-    for (int idx = threadIdx.x; idx < TY * TX; idx += blockDim.x) {
-        int oy = idx / TX, ox = idx % TX;
-        int gy = ty0 + oy, gx = tx0 + ox;
-        if (gy >= rh || gx >= rw) continue;
-        out[(size_t)gy * S.out_stride + gx] = s[HALO + oy][HALO + ox];
+        if (gx < rw) {
+#pragma unroll
+            for (int j = 0; j < SEG; j++) {
+                const int gy = ty0 + g * SEG + j;
+                if (gy < rh) out[(size_t)gy * S.out_stride + gx] = v[HALO + j];
+            }
+        }
     }
 }
 
@@ -403,23 +411,16 @@ __global__ __launch_bounds__(256) void finalize_kernel(const PixSeg* __restrict_
 }
 
 // ---- host launchers ---------------------------------------------------------------
-// A level with few tiles lasts about as long as one tile's chain of lifting
-// steps and barriers, so its tiles get 1024 threads, which makes each
-// thread's share of a step 4x smaller.  Levels of 1024 tiles or more keep
-// 256 threads: they are throughput-bound (a 16-page batch's DWT measured
-// 1.24 ms with 256 there, 1.30 ms with 1024).
-inline int level_threads(uint32_t ntiles) { return ntiles < 1024 ? 1024 : 256; }
+
 void launch_fwd_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles, hipStream_t st) {
     if (!ntiles) return;
-    const int nt = level_threads(ntiles);
-    if (irrev) fwd_level_kernel<float, true><<<ntiles, nt, 0, st>>>(segs, nseg);
-    else fwd_level_kernel<int32_t, false><<<ntiles, nt, 0, st>>>(segs, nseg);
+    if (irrev) fwd_level_kernel<float, true><<<ntiles, kLevelThreads, 0, st>>>(segs, nseg);
+    else fwd_level_kernel<int32_t, false><<<ntiles, kLevelThreads, 0, st>>>(segs, nseg);
 }
 void launch_inv_level(bool irrev, const DwtSeg* segs, int nseg, uint32_t ntiles, hipStream_t st) {
     if (!ntiles) return;
-    const int nt = level_threads(ntiles);
-    if (irrev) inv_level_kernel<float, true><<<ntiles, nt, 0, st>>>(segs, nseg);
-    else inv_level_kernel<int32_t, false><<<ntiles, nt, 0, st>>>(segs, nseg);
+    if (irrev) inv_level_kernel<float, true><<<ntiles, kLevelThreads, 0, st>>>(segs, nseg);
+    else inv_level_kernel<int32_t, false><<<ntiles, kLevelThreads, 0, st>>>(segs, nseg);
 }
 void launch_prepare(const PixSeg* segs, int nseg, uint32_t nblocks, hipStream_t st) {
     if (nblocks) prepare_kernel<<<nblocks, 256, 0, st>>>(segs, nseg);
