@@ -312,102 +312,188 @@ __device__ __forceinline__ int find_pix(const PixSeg* segs, int nseg, uint32_t b
     return lo;
 }
 
+// Pixel kernels: a thread takes 4 consecutive pixels (a wave 256), read
+// and written with vector accesses when the pointers allow it, else one
+// element at a time.
+__device__ __forceinline__ bool aligned_to(uint64_t p, uint32_t a) { return (p & (a - 1u)) == 0; }
+
+// One transformed sample (scalar.rs:135-195): level shift, forward ICT
+// evaluated left to right without contraction, or forward RCT.
+__device__ __forceinline__ float ict_sample(int comp, int r, int g, int b) {
+    const float rf = (float)(r - 128), gf = (float)(g - 128), bf = (float)(b - 128);
+    if (comp == 0) return __fadd_rn(__fadd_rn(__fmul_rn(0.299f, rf), __fmul_rn(0.587f, gf)), __fmul_rn(0.114f, bf));
+    if (comp == 1) return __fadd_rn(__fadd_rn(__fmul_rn(-0.16875f, rf), __fmul_rn(-0.33126f, gf)), __fmul_rn(0.5f, bf));
+    return __fsub_rn(__fsub_rn(__fmul_rn(0.5f, rf), __fmul_rn(0.41869f, gf)), __fmul_rn(0.08131f, bf));
+}
+__device__ __forceinline__ int rct_sample(int comp, int r, int g, int b) {
+    const int rv = r - 128, gv = g - 128, bv = b - 128;
+    return comp == 0 ? ((rv + 2 * gv + bv) >> 2) : comp == 1 ? (bv - gv) : (rv - gv);
+}
+
 // Level shift (scalar.rs:242-254), forward ICT (scalar.rs:256-278) and
-// forward RCT (scalar.rs:280-302) into a component plane.
-__global__ __launch_bounds__(256) void prepare_kernel(const PixSeg* __restrict__ segs, int nseg) {
-    const PixSeg S = segs[find_pix(segs, nseg, blockIdx.x)];
-    const size_t i0 = (size_t)(blockIdx.x - S.blk_start) * kPixPerBlock + threadIdx.x;
-    for (uint32_t c256 = 0; c256 < kPixPerBlock / 256u; c256++) {
-    const size_t i = i0 + (size_t)c256 * 256u;  // each pass a coalesced 256
-    if (i >= S.n) return;
-    int r, g, b;
+// forward RCT (scalar.rs:280-302) into a component plane, for ch channels
+// (1 or 3) of input.
+template <int ch>
+__device__ __forceinline__ void prepare4(const PixSeg& S, uint64_t i) {
+    const bool full = i + 4u <= S.n;
+    int r[4], g[4], b[4];
     if (S.planar_i32) {
         const int32_t* p = (const int32_t*)S.src;
-        if (S.channels == 1) {
-            r = g = b = p[i];
-        } else {
-            r = p[i];
-            g = p[S.n + i];
-            b = p[2 * S.n + i];
+#pragma unroll
+        for (int c = 0; c < ch; c++) {
+            int* d = c == 0 ? r : c == 1 ? g : b;
+            const int32_t* q = p + (uint64_t)c * S.n + i;
+            if (full && aligned_to((uint64_t)q, 16)) {
+                const int4 v = *(const int4*)q;
+                d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) d[k] = i + k < S.n ? q[k] : 0;
+            }
         }
     } else {
-        const uint8_t* p = (const uint8_t*)S.src;
-        if (S.channels == 1) {
-            r = g = b = p[i];
+        const uint8_t* q = (const uint8_t*)S.src + i * (uint64_t)ch;
+        uint32_t w[3] = {0, 0, 0};  // the 4 pixels' bytes, in order
+        if (full && aligned_to((uint64_t)q, 4)) {
+#pragma unroll
+            for (int k = 0; k < ch; k++) w[k] = ((const uint32_t*)q)[k];
         } else {
-            r = p[3 * i];
-            g = p[3 * i + 1];
-            b = p[3 * i + 2];
+            for (int k = 0; k < 4 * ch; k++)
+                if (i * ch + k < S.n * ch) w[k >> 2] |= (uint32_t)q[k] << (8 * (k & 3));
         }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (ch == 1) {
+                r[k] = (int)((w[0] >> (8 * k)) & 255u);
+            } else {
+                const int o = 3 * k;
+                r[k] = (int)((w[o >> 2] >> (8 * (o & 3))) & 255u);
+                g[k] = (int)((w[(o + 1) >> 2] >> (8 * ((o + 1) & 3))) & 255u);
+                b[k] = (int)((w[(o + 2) >> 2] >> (8 * ((o + 2) & 3))) & 255u);
+            }
+        }
+    }
+    if (ch == 1) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) g[k] = b[k] = r[k];
     }
     if (S.irrev) {
-        float v;
-        if (S.channels == 1) {
-            v = (float)(r - 128);
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = ch == 1 ? (float)(r[k] - 128) : ict_sample(S.comp, r[k], g[k], b[k]);
+        float* d = (float*)S.dst + i;
+        if (full && aligned_to((uint64_t)d, 16)) {
+            *(float4*)d = make_float4(v[0], v[1], v[2], v[3]);
         } else {
-            float rf = (float)(r - 128), gf = (float)(g - 128), bf = (float)(b - 128);
-            if (S.comp == 0) v = __fadd_rn(__fadd_rn(__fmul_rn(0.299f, rf), __fmul_rn(0.587f, gf)), __fmul_rn(0.114f, bf));
-            else if (S.comp == 1)
-                v = __fadd_rn(__fadd_rn(__fmul_rn(-0.16875f, rf), __fmul_rn(-0.33126f, gf)), __fmul_rn(0.5f, bf));
-            else v = __fsub_rn(__fsub_rn(__fmul_rn(0.5f, rf), __fmul_rn(0.41869f, gf)), __fmul_rn(0.08131f, bf));
+            for (int k = 0; k < 4; k++)
+                if (i + k < S.n) d[k] = v[k];
         }
-        ((float*)S.dst)[i] = v;
     } else {
-        int v;
-        if (S.channels == 1) {
-            v = r - 128;
+        int v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = ch == 1 ? r[k] - 128 : rct_sample(S.comp, r[k], g[k], b[k]);
+        int32_t* d = (int32_t*)S.dst + i;
+        if (full && aligned_to((uint64_t)d, 16)) {
+            *(int4*)d = make_int4(v[0], v[1], v[2], v[3]);
         } else {
-            int rv = r - 128, gv = g - 128, bv = b - 128;
-            v = S.comp == 0 ? ((rv + 2 * gv + bv) >> 2) : S.comp == 1 ? (bv - gv) : (rv - gv);
+            for (int k = 0; k < 4; k++)
+                if (i + k < S.n) d[k] = v[k];
         }
-        ((int32_t*)S.dst)[i] = v;
     }
-    }
+}
+__global__ __launch_bounds__(256) void prepare_kernel(const PixSeg* __restrict__ segs, int nseg) {
+    const PixSeg S = segs[find_pix(segs, nseg, blockIdx.x)];
+    const uint64_t i = (uint64_t)(blockIdx.x - S.blk_start) * kPixPerBlock + 4u * threadIdx.x;
+    if (i >= S.n) return;
+    if (S.channels == 3) prepare4<3>(S, i);
+    else prepare4<1>(S, i);
 }
 
 // Inverse ICT/RCT and round/clamp to 0..255 (scalar.rs:197-236; f32::round
-// is half away from zero).
+// is half away from zero), for CH channels (1 or 3).
+template <int CH>
+__device__ __forceinline__ void finalize4(const PixSeg& S, uint64_t i) {
+    const bool full = i + 4u <= S.n;
+    // the component planes' 4 samples (f32 bits or i32)
+    uint32_t x[CH][4];
+    const uint64_t src[3] = {S.src, S.src1, S.src2};
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        const uint32_t* q = (const uint32_t*)src[c] + i;
+        if (full && aligned_to((uint64_t)q, 16)) {
+            const uint4 v = *(const uint4*)q;
+            x[c][0] = v.x, x[c][1] = v.y, x[c][2] = v.z, x[c][3] = v.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) x[c][k] = i + k < S.n ? q[k] : 0u;
+        }
+    }
+    int v[4][CH];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (S.irrev) {
+            float f[CH];
+            const float y = __uint_as_float(x[0][k]);
+            if constexpr (CH == 3) {
+                const float cb = __uint_as_float(x[1][k]), cr = __uint_as_float(x[2][k]);
+                f[0] = __fadd_rn(y, __fmul_rn(1.402f, cr));
+                f[1] = __fsub_rn(__fsub_rn(y, __fmul_rn(0.34413f, cb)), __fmul_rn(0.71414f, cr));
+                f[2] = __fadd_rn(y, __fmul_rn(1.772f, cb));
+            } else {
+                f[0] = y;
+            }
+#pragma unroll
+            for (int c = 0; c < CH; c++) v[k][c] = (int)fminf(fmaxf(roundf(__fadd_rn(f[c], 128.0f)), 0.0f), 255.0f);
+        } else {
+            const int y = (int)x[0][k];
+            int t[CH];
+            if constexpr (CH == 3) {
+                const int db = (int)x[1][k], dr = (int)x[2][k];
+                const int g = y - ((db + dr) >> 2);
+                t[0] = dr + g;
+                t[1] = g;
+                t[2] = db + g;
+            } else {
+                t[0] = y;
+            }
+#pragma unroll
+            for (int c = 0; c < CH; c++) v[k][c] = min(max(t[c] + 128, 0), 255);
+        }
+    }
+    if (S.dst) {  // u8 interleaved: 4 * CH bytes
+        uint8_t* o = (uint8_t*)S.dst + i * CH;
+        uint32_t w[CH];
+#pragma unroll
+        for (int k = 0; k < CH; k++) w[k] = 0;
+#pragma unroll
+        for (int k = 0; k < 4 * CH; k++) w[k >> 2] |= (uint32_t)v[k / CH][k % CH] << (8 * (k & 3));
+        if (full && aligned_to((uint64_t)o, 4)) {
+#pragma unroll
+            for (int k = 0; k < CH; k++) ((uint32_t*)o)[k] = w[k];
+        } else {
+            for (int k = 0; k < 4 * CH; k++)
+                if (i * CH + k < S.n * CH) o[k] = (uint8_t)(w[k >> 2] >> (8 * (k & 3)));
+        }
+    }
+    if (S.dst_i32) {  // i32 planar
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            int32_t* o = (int32_t*)S.dst_i32 + (uint64_t)c * S.n + i;
+            if (full && aligned_to((uint64_t)o, 16)) {
+                *(int4*)o = make_int4(v[0][c], v[1][c], v[2][c], v[3][c]);
+            } else {
+                for (int k = 0; k < 4; k++)
+                    if (i + k < S.n) o[k] = v[k][c];
+            }
+        }
+    }
+}
 __global__ __launch_bounds__(256) void finalize_kernel(const PixSeg* __restrict__ segs, int nseg) {
     const PixSeg S = segs[find_pix(segs, nseg, blockIdx.x)];
-    const size_t i0 = (size_t)(blockIdx.x - S.blk_start) * kPixPerBlock + threadIdx.x;
-    for (uint32_t c256 = 0; c256 < kPixPerBlock / 256u; c256++) {
-    const size_t i = i0 + (size_t)c256 * 256u;  // each pass a coalesced 256
+    const uint64_t i = (uint64_t)(blockIdx.x - S.blk_start) * kPixPerBlock + 4u * threadIdx.x;
     if (i >= S.n) return;
-    const int ch = S.channels;
-    int v[3];
-    if (S.irrev) {
-        float f[3];
-        float y = ((const float*)S.src)[i];
-        if (ch == 3) {
-            float cb = ((const float*)S.src1)[i], cr = ((const float*)S.src2)[i];
-            f[0] = __fadd_rn(y, __fmul_rn(1.402f, cr));
-            f[1] = __fsub_rn(__fsub_rn(y, __fmul_rn(0.34413f, cb)), __fmul_rn(0.71414f, cr));
-            f[2] = __fadd_rn(y, __fmul_rn(1.772f, cb));
-        } else {
-            f[0] = y;
-        }
-        for (int c = 0; c < ch; c++) v[c] = (int)fminf(fmaxf(roundf(__fadd_rn(f[c], 128.0f)), 0.0f), 255.0f);
-    } else {
-        int y = ((const int32_t*)S.src)[i];
-        int t[3];
-        if (ch == 3) {
-            int db = ((const int32_t*)S.src1)[i], dr = ((const int32_t*)S.src2)[i];
-            int g = y - ((db + dr) >> 2);
-            t[0] = dr + g;
-            t[1] = g;
-            t[2] = db + g;
-        } else {
-            t[0] = y;
-        }
-        for (int c = 0; c < ch; c++) v[c] = min(max(t[c] + 128, 0), 255);
-    }
-    uint8_t* o8 = (uint8_t*)S.dst;
-    int32_t* o32 = (int32_t*)S.dst_i32;
-    for (int c = 0; c < ch; c++) {
-        if (o8) o8[i * ch + c] = (uint8_t)v[c];
-        if (o32) o32[(size_t)c * S.n + i] = v[c];
-    }
-    }
+    if (S.channels == 3) finalize4<3>(S, i);
+    else finalize4<1>(S, i);
 }
 
 // ---- host launchers ---------------------------------------------------------------
