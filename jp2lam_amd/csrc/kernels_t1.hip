@@ -1580,8 +1580,11 @@ __global__ __launch_bounds__(1024) void t1_split_kernel(int nb, const BlockResul
     }
 }
 
-// Workgroups [0, nb): the scalar coder on long stream i (if there is one);
-// then one lane-coder wave per 64 short streams.
+// Workgroups [0, ceil(nb / 64)): one lane-coder wave per 64 short streams
+// (if there are that many); then [.., + nb): the scalar coder on long
+// stream i (if there is one).  The grid is sized for any split, and
+// workgroups start in index order, so the lane waves and the long streams
+// come first and the surplus workgroups, which exit at once, last.
 __global__ __launch_bounds__(64) void t1_hybrid_kernel(int nb, const BlockJob* __restrict__ jobs,
                                                        const uint32_t* __restrict__ tabs, const uint8_t* __restrict__ pool,
                                                        const uint32_t* __restrict__ soff16,
@@ -1590,14 +1593,16 @@ __global__ __launch_bounds__(64) void t1_hybrid_kernel(int nb, const BlockJob* _
                                                        const uint32_t* __restrict__ lists) {
     __shared__ uint32_t cst[32 * 64];
     __shared__ uint32_t tab[64];
-    if ((int)blockIdx.x < nb) {
-        if (blockIdx.x >= lists[0]) return;
-        stream_mq((int)lists[2 + blockIdx.x], jobs, tabs, pool, soff16, pend, results, snaps, out);
-    } else {
-        const uint32_t g0 = (blockIdx.x - (uint32_t)nb) * 64u, nshort = lists[1];
+    const uint32_t nlane = ((uint32_t)nb + 63u) / 64u;
+    if (blockIdx.x < nlane) {
+        const uint32_t g0 = blockIdx.x * 64u, nshort = lists[1];
         if (g0 >= nshort) return;
         const uint32_t j = g0 + lid();
         lane_mq(j < nshort ? (int)lists[2 + nb + j] : -1, jobs, tabs, pool, soff16, pend, results, snaps, out, cst, tab);
+    } else {
+        const uint32_t i = blockIdx.x - nlane;
+        if (i >= lists[0]) return;
+        stream_mq((int)lists[2 + i], jobs, tabs, pool, soff16, pend, results, snaps, out);
     }
 }
 
