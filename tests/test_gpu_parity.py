@@ -535,3 +535,37 @@ def test_hidden_colour_spaces_match_oracle(gpu, ora, cs, q):
     img = j.Image(w, h, [j.Component(planes[c].reshape(-1), w, h) for c in range(3)], cs)
     ours = j.encode(img, j.EncodeOptions(q))
     assert ours == _oracle_encode_planes(ora, planes.reshape(3, -1), w, h, int(cs), q)
+
+
+@pytest.mark.parametrize("w,h,ch,q", [(131, 97, 1, 85), (131, 97, 1, 100), (45, 61, 3, 75), (45, 61, 3, 100)])
+def test_pixel_kernels_unaligned_device_buffers(gpu, ora, lib, w, h, ch, q):
+    """Device input and output at byte offsets 0-3 and pixel counts that are
+    not multiples of four: the colour and finalize kernels' element-wise
+    paths give the same codestream and pixels as their vector paths, and
+    nothing is written outside the output."""
+    px = synth.document_page(w, h, seed=5) if ch == 1 else synth.photo_page(w, h, seed=5)
+    n = px.size
+    assert n % 4 != 0
+    want = ora.encode(px, q)
+    # sRGB lossless: the reference rejects its own stream (chroma Mb)
+    want_px = None if (ch == 3 and q >= 100) else ora.decode(want)
+    d_in = lib.jp2lam_device_alloc(n + 16)
+    d_out = lib.jp2lam_device_alloc(n + 16)
+    try:
+        zeros = np.zeros(n + 16, np.uint8)
+        for off in range(4):
+            assert lib.jp2lam_device_copy(d_in + off, px.ctypes.data, n, 0) == 0
+            got = j.encode_many([((d_in + off, h, w, ch), q)], device=True)[0]
+            assert got == want, f"offset {off}"
+            if want_px is None:
+                continue
+            assert lib.jp2lam_device_copy(d_out, zeros.ctypes.data, n + 16, 0) == 0
+            res = j.decode_many([got], device_outputs=[(d_out + off, n)])[0]
+            assert not isinstance(res, Exception)
+            back = np.empty(n + 16, np.uint8)
+            assert lib.jp2lam_device_copy(back.ctypes.data, d_out, n + 16, 1) == 0
+            assert np.array_equal(back[off:off + n].reshape(px.shape), want_px), f"offset {off}"
+            assert not back[:off].any() and not back[off + n:].any(), f"offset {off}: write outside the output"
+    finally:
+        lib.jp2lam_device_free(d_in)
+        lib.jp2lam_device_free(d_out)
