@@ -4,6 +4,7 @@
 #include "engine.hpp"
 
 #include <functional>
+#include <memory>
 
 #include <algorithm>
 #include <atomic>
@@ -189,7 +190,17 @@ void GraphCache::run(hipStream_t st, const std::function<void(bool, std::vector<
 Engine::~Engine() {
     graphs_.clear();
     if (tables_) (void)hipFree(tables_);
+    if (ev_up_) (void)hipEventDestroy(ev_up_);
+    if (ev_side_) (void)hipEventDestroy(ev_side_);
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::ensure_side_stream() {
+    if (side_) return;
+    J2L_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    J2L_HIP(hipEventCreateWithFlags(&ev_up_, hipEventDisableTiming));
+    J2L_HIP(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
 }
 
 namespace {
@@ -1048,11 +1059,44 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     // one stream at a time).
     std::vector<Img> parsed(items.size());
     std::vector<char> ok(items.size(), 0);
+    // A single image of four or more resolutions is decoded in two parts:
+    // the packets of all but the two finest resolutions are parsed first and
+    // their blocks start on the device (they hold the longest segments);
+    // the rest are parsed meanwhile and coded on the side stream.  If the
+    // first part's blocks fail a check, the image is parsed and checked
+    // whole, as below, so the error is the one the whole parse reports.
+    std::unique_ptr<PacketParser> split_pp;
+    int split_res = -1;
     auto parse_one = [&](size_t i) {
         try {
             Img& im = parsed[i];
             im.idx = (int)i;
-            parse_for_decode(items[i].bytes, items[i].len, im.jp, im.ci, im.blocks);
+            if (items.size() == 1) {
+                parse_decode_header(items[i].bytes, items[i].len, im.jp, im.ci);
+                auto pp = std::make_unique<PacketParser>(im.ci, im.blocks);
+                bool split = im.ci.levels >= 3;
+                if (split) {
+                    pp->run(im.ci.levels - 2);
+                    try {
+                        std::vector<DecBlock> first;
+                        for (const auto& b : im.blocks)
+                            if (b.res <= im.ci.levels - 2) first.push_back(b);
+                        check_decode_blocks(im.ci, first);
+                    } catch (const Error&) {
+                        split = false;
+                    }
+                }
+                if (split) {
+                    split_pp = std::move(pp);
+                    split_res = im.ci.levels - 2;
+                } else {
+                    pp->run(im.ci.levels);
+                    pp->finish();
+                    check_decode_blocks(im.ci, im.blocks);
+                }
+            } else {
+                parse_for_decode(items[i].bytes, items[i].len, im.jp, im.ci, im.blocks);
+            }
             im.n = (size_t)im.ci.width * im.ci.height;
             resolution_ladder(im.ci.width, im.ci.height, im.ci.levels, im.rw, im.rh);
             im.pl = plane_layout(im.ci.width, im.ci.levels, false);
@@ -1093,7 +1137,51 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     for (size_t i = 0; i < items.size(); i++)
         if (ok[i]) imgs.push_back(std::move(parsed[i]));
     if (imgs.empty()) return outs;
+    if (split_pp) split_pp->rebind(imgs[0].ci, imgs[0].blocks);
 
+    // one block's Tier-1 job; its plane is the component index until the
+    // arena is placed
+    auto make_job = [](const Img& im, const DecBlock& b) {
+        DecodeJob j{};
+        j.plane = (uint64_t)b.comp;
+        j.stride = im.pl.stride;
+        const bool detail_x = b.band == HL || b.band == HH;
+        j.x0 = (uint16_t)(detail_x ? im.pl.hx[b.res] + (uint32_t)b.r.x0 - im.rw[b.res - 1] : (uint32_t)b.r.x0);
+        j.y0 = (uint16_t)b.r.y0;
+        j.w = (uint8_t)(b.r.x1 - b.r.x0);
+        j.h = (uint8_t)(b.r.y1 - b.r.y0);
+        j.band = (uint8_t)b.band;
+        const int si = b.band_index % (int)im.ci.exps.size();
+        j.mb = (uint8_t)((std::max(im.ci.guard - 1, 0) + im.ci.exps[si]) & 255);
+        // decode/reconstruct.rs:202-231: step from the component precision
+        j.step = im.ci.transform == 0
+                     ? quant_step_f32(b.band, im.ci.exps[si], im.ci.mants[si], (uint32_t)im.ci.prec[b.comp])
+                     : 0.0f;
+        j.data_off = (uint32_t)(im.pay_off + b.off);
+        j.data_len = b.len;
+        j.zbp = (int16_t)(b.included && b.passes > 0 ? b.zbp : -1);
+        j.passes = (uint8_t)std::min(b.passes, 255);
+        return j;
+    };
+    // issue priority by segment length rank: the kernel ends with its
+    // heaviest block, so those blocks win their SIMD's issue slots.  Levels
+    // 3 / 2 / 1 go to the longest 2% / 10% / 30% of a table (f = 1); the
+    // first part of a split decode holds the coarse resolutions, where the
+    // longest segments are, and ranks with f = 15 (30% / 100% / all).
+    auto rank_jobs = [](DecodeJob* jb, int n, size_t f) {
+        if (n <= 0) return;
+        std::vector<uint32_t> sorted(n);
+        for (int k = 0; k < n; k++) sorted[k] = jb[k].data_len;
+        std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
+        const size_t m = (size_t)n - 1;
+        const uint32_t t3 = sorted[std::min(m, (size_t)n * f / 50)];
+        const uint32_t t2 = sorted[std::min(m, (size_t)n * f / 10)];
+        const uint32_t t1 = sorted[std::min(m, (size_t)n * f * 3 / 10)];
+        for (int k = 0; k < n; k++) {
+            const uint32_t l = jb[k].data_len;
+            jb[k].pad = (uint8_t)(l >= t3 && l ? 3 : l >= t2 && l ? 2 : l >= t1 && l ? 1 : 0);
+        }
+    };
     Layout L;
     std::vector<DecodeJob> jobs;
     size_t pay_total = 0;
@@ -1121,29 +1209,12 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         im.u8_off = items[im.idx].target == DST_DEV_U8 ? 0 : L.add(im.n * nc);
         im.job_first = (uint32_t)jobs.size();
         maxL = std::max(maxL, im.ci.levels);
-        const bool irrev = im.ci.transform == 0;
-        for (auto& b : im.blocks) {
-            DecodeJob j{};
-            j.plane = (uint64_t)b.comp;  // rebased below
-            j.stride = im.pl.stride;
-            const bool detail_x = b.band == HL || b.band == HH;
-            j.x0 = (uint16_t)(detail_x ? im.pl.hx[b.res] + (uint32_t)b.r.x0 - im.rw[b.res - 1] : (uint32_t)b.r.x0);
-            j.y0 = (uint16_t)b.r.y0;
-            j.w = (uint8_t)(b.r.x1 - b.r.x0);
-            j.h = (uint8_t)(b.r.y1 - b.r.y0);
-            j.band = (uint8_t)b.band;
-            const int si = b.band_index % (int)im.ci.exps.size();
-            j.mb = (uint8_t)((std::max(im.ci.guard - 1, 0) + im.ci.exps[si]) & 255);
-            // decode/reconstruct.rs:202-231: step from the component precision
-            j.step = irrev ? quant_step_f32(b.band, im.ci.exps[si], im.ci.mants[si], (uint32_t)im.ci.prec[b.comp]) : 0.0f;
-            j.data_off = (uint32_t)(im.pay_off + b.off);
-            j.data_len = b.len;
-            j.zbp = (int16_t)(b.included && b.passes > 0 ? b.zbp : -1);
-            j.passes = (uint8_t)std::min(b.passes, 255);
-            jobs.push_back(j);
-        }
+        for (auto& b : im.blocks)
+            if (split_res < 0 || b.res <= split_res) jobs.push_back(make_job(im, b));  // the rest: parsed later
     }
     const int nj = (int)jobs.size();
+    // a split decode's table holds every block: its second part follows
+    const int nj_all = split_res >= 0 ? (int)imgs[0].blocks.size() : nj;
     // inverse segment tables per step k (coarsest first)
     std::vector<std::vector<DwtSeg>> isegs(2 * std::max(maxL, 1));
     std::vector<PixSeg> fsegs[2];
@@ -1187,7 +1258,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     // jobs and segment tables follow each other in their staging order, so
     // their uploads merge into one copy (an empty table takes no space; its
     // launch is skipped)
-    const size_t o_jobs = L.add(sizeof(DecodeJob) * nj);
+    const size_t o_jobs = L.add(sizeof(DecodeJob) * nj_all);
     std::vector<size_t> o_iseg(isegs.size());
     for (size_t q = 0; q < isegs.size(); q++) o_iseg[q] = L.add(sizeof(DwtSeg) * isegs[q].size());
     const size_t o_fseg0 = L.add(sizeof(PixSeg) * std::max<size_t>(fsegs[0].size(), 1));
@@ -1197,7 +1268,7 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     char* D = dev_.base;
     auto A = [&](size_t off) { return (uint64_t)(uintptr_t)(D + off); };
     for (auto& im : imgs)
-        for (uint32_t j = im.job_first; j < im.job_first + (uint32_t)im.blocks.size(); j++)
+        for (uint32_t j = im.job_first; j < std::min<size_t>(im.job_first + im.blocks.size(), jobs.size()); j++)
             jobs[j].plane = A(im.p_off[jobs[j].plane]);
     std::vector<uint32_t> itiles(isegs.size(), 0);
     for (size_t q = 0; q < isegs.size(); q++)
@@ -1237,30 +1308,61 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     };
     for (auto& im : imgs) stage(o_pay + im.pay_off, im.ci.payload, im.ci.payload_len);
     if (nj) {
-        // issue priority by segment length rank: the kernel ends with its
-        // heaviest block, so those blocks win their SIMD's issue slots
-        std::vector<uint32_t> lens(nj);
-        for (int k = 0; k < nj; k++) lens[k] = jobs[k].data_len;
-        std::vector<uint32_t> sorted = lens;
-        std::sort(sorted.begin(), sorted.end(), std::greater<uint32_t>());
-        const uint32_t t3 = sorted[std::min<size_t>(nj - 1, (size_t)nj / 50)];
-        const uint32_t t2 = sorted[std::min<size_t>(nj - 1, (size_t)nj / 10)];
-        const uint32_t t1 = sorted[std::min<size_t>(nj - 1, (size_t)nj * 3 / 10)];
-        for (int k = 0; k < nj; k++)
-            jobs[k].pad = (uint8_t)(lens[k] >= t3 && lens[k] ? 3 : lens[k] >= t2 && lens[k] ? 2 : lens[k] >= t1 && lens[k] ? 1 : 0);
+        rank_jobs(jobs.data(), nj, split_res >= 0 ? 15 : 1);
         stage(o_jobs, jobs.data(), sizeof(DecodeJob) * nj);
     }
     for (size_t q = 0; q < isegs.size(); q++) stage(o_iseg[q], isegs[q].data(), sizeof(DwtSeg) * isegs[q].size());
     stage(o_fseg0, fsegs[0].data(), sizeof(PixSeg) * fsegs[0].size());
     stage(o_fseg1, fsegs[1].data(), sizeof(PixSeg) * fsegs[1].size());
+    // a split decode stages its second job table after these (the first
+    // uploads may still be reading the arena then)
+    const size_t h_rest = H.add(sizeof(DecodeJob) * (size_t)(nj_all - nj));
     pin_.reserve(H.used);
     copy_staged(staged, D, pin_.base, st);
 
     Events& ev = ev_;
-    // Tier-1 is one launch; the inverse DWT levels and the pixel output run
-    // as one graph (or their launches), between the stage events.
+    // Tier-1 is one launch (a split decode: one per part, the second on the
+    // side stream); the inverse DWT levels and the pixel output run as one
+    // graph (or their launches), between the stage events.
+    if (split_res >= 0) {
+        ensure_side_stream();
+        J2L_HIP(hipEventRecord(ev_up_, st));  // the payload is on the device
+    }
     J2L_HIP(hipEventRecord(ev.e[0], st));
     launch_t1_decode((const DecodeJob*)(D + o_jobs), nj, tables_, (const uint8_t*)(D + o_pay), st);
+    if (split_res >= 0) {
+        // the finest resolutions' packets, parsed while the first part runs
+        Img& im = imgs[0];
+        try {
+            split_pp->run(im.ci.levels);
+            split_pp->finish();
+            check_decode_blocks(im.ci, im.blocks);
+        } catch (const Error& e) {
+            J2L_HIP(hipStreamSynchronize(st));
+            outs[im.idx].status = e.status;
+            outs[im.idx].message = e.what();
+            return outs;
+        }
+        std::vector<DecodeJob> rest;
+        rest.reserve(nj_all - nj);
+        for (auto& b : im.blocks)
+            if (b.res > split_res) {
+                DecodeJob j = make_job(im, b);
+                j.plane = A(im.p_off[j.plane]);
+                rest.push_back(j);
+            }
+        const int nr = (int)rest.size();
+        if (nr) {
+            rank_jobs(rest.data(), nr, 1);
+            std::memcpy(pin_.base + h_rest, rest.data(), sizeof(DecodeJob) * nr);
+            J2L_HIP(hipStreamWaitEvent(side_, ev_up_, 0));
+            J2L_HIP(hipMemcpyAsync(D + o_jobs + sizeof(DecodeJob) * nj, pin_.base + h_rest, sizeof(DecodeJob) * nr,
+                                   hipMemcpyHostToDevice, side_));
+            launch_t1_decode((const DecodeJob*)(D + o_jobs) + nj, nr, tables_, (const uint8_t*)(D + o_pay), side_);
+            J2L_HIP(hipEventRecord(ev_side_, side_));
+            J2L_HIP(hipStreamWaitEvent(st, ev_side_, 0));
+        }
+    }
     J2L_HIP(hipEventRecord(ev.e[1], st));
     graphs_.run(st, [&](bool go, std::vector<uint64_t>& sig) {
         sig.insert(sig.end(), {9, (uint64_t)(uintptr_t)D, (uint64_t)L.used, o_fseg0, o_fseg1, fsegs[0].size(),

@@ -153,7 +153,13 @@ public:
     std::mutex mu;
 
 private:
-    hipStream_t stream_ = nullptr;
+    // A second stream and two events (no timing) for a single-image decode:
+    // the packets of its finest resolutions are parsed while Tier-1 decodes
+    // the coarser ones, whose blocks are then coded on this stream beside
+    // them (created on first use).
+    void ensure_side_stream();
+    hipStream_t stream_ = nullptr, side_ = nullptr;
+    hipEvent_t ev_up_ = nullptr, ev_side_ = nullptr;
     Arena dev_, pin_;
     Events ev_;
     GraphCache graphs_;

@@ -846,12 +846,12 @@ struct BitReader {
 };
 }  // namespace
 
-void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
-    struct BandState {
-        int comp, res, band, first, cx, cy;
-        TagTree incl, imsb;
-    };
-    std::vector<BandState> bands;
+struct PacketParser::Band {
+    int comp, res, band, first, cx, cy;
+    TagTree incl, imsb;
+};
+
+PacketParser::PacketParser(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) : ci_(&ci), blocks_(&blocks) {
     blocks.clear();
     const int L = ci.levels;
     for (int c = 0; c < ci.ncomp; c++)
@@ -860,9 +860,9 @@ void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
                 Rect rc = band_rect(ci.width, ci.height, L, r, bd);
                 int cx = rc.x1 > rc.x0 ? (rc.x1 - rc.x0 + ci.cbw - 1) / ci.cbw : 0;
                 int cy = rc.y1 > rc.y0 ? (rc.y1 - rc.y0 + ci.cbh - 1) / ci.cbh : 0;
-                int bi = (int)bands.size();
-                bands.push_back({c, r, bd, (int)blocks.size(), cx, cy, TagTree(std::max(cx, 1), std::max(cy, 1), 0x7fffffff),
-                                 TagTree(std::max(cx, 1), std::max(cy, 1), 0x7fffffff)});
+                int bi = (int)bands_.size();
+                bands_.push_back({c, r, bd, (int)blocks.size(), cx, cy, TagTree(std::max(cx, 1), std::max(cy, 1), 0x7fffffff),
+                                  TagTree(std::max(cx, 1), std::max(cy, 1), 0x7fffffff)});
                 for (int by = rc.y0; by < rc.y1; by += ci.cbh)
                     for (int bx = rc.x0; bx < rc.x1; bx += ci.cbw) {
                         DecBlock db;
@@ -874,21 +874,29 @@ void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
                         blocks.push_back(db);
                     }
             }
+}
+PacketParser::~PacketParser() = default;
+
+void PacketParser::run(int rmax) {
+    const CodestreamInfo& ci = *ci_;
+    std::vector<DecBlock>& blocks = *blocks_;
     const uint8_t* pay = ci.payload;
-    size_t plen = ci.payload_len, pos = 0;
+    const size_t plen = ci.payload_len;
     std::vector<int> order;
-    for (int layer = 0; layer < ci.layers; layer++)
-        for (int r = 0; r <= L; r++)
+    // one layer (checked with the header), so LRCP is resolution-major
+    for (; layer_ < ci.layers; layer_++, res_ = 0)
+        for (; res_ <= ci.levels; res_++) {
+            if (res_ > rmax) return;
             for (int c = 0; c < ci.ncomp; c++) {
-                BitReader br{pay + pos, plen - pos};
+                BitReader br{pay + pos_, plen - pos_};
                 order.clear();
                 uint64_t body = 0;
                 if (br.bit()) {
-                    for (auto& bs : bands) {
-                        if (bs.comp != c || bs.res != r) continue;
+                    for (auto& bs : bands_) {
+                        if (bs.comp != c || bs.res != res_) continue;
                         for (int k = 0; k < bs.cx * bs.cy; k++) {
                             DecBlock& blk = blocks[bs.first + k];
-                            bool inc = blk.included ? br.bit() != 0 : bs.incl.decode(br, k, layer + 1);
+                            bool inc = blk.included ? br.bit() != 0 : bs.incl.decode(br, k, layer_ + 1);
                             if (!inc) continue;
                             if (blk.zbp < 0) {
                                 bs.imsb.decode(br, k, 999);
@@ -919,14 +927,25 @@ void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
                         }
                     }
                 }
-                pos += br.pos;
-                if (body > plen - pos) fail("packet body extends past tile payload");
+                pos_ += br.pos;
+                if (body > plen - pos_) fail("packet body extends past tile payload");
                 for (int i : order) {
-                    blocks[i].off = (uint32_t)pos;
-                    pos += blocks[i].len;
+                    blocks[i].off = (uint32_t)pos_;
+                    pos_ += blocks[i].len;
                 }
             }
-    if (pos != plen) fail("tile payload has " + std::to_string(plen - pos) + " trailing bytes after packet parse");
+        }
+}
+
+void PacketParser::finish() {
+    if (pos_ != ci_->payload_len)
+        fail("tile payload has " + std::to_string(ci_->payload_len - pos_) + " trailing bytes after packet parse");
+}
+
+void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
+    PacketParser pp(ci, blocks);
+    pp.run(ci.levels);
+    pp.finish();
 }
 
 
@@ -953,7 +972,7 @@ void check_decode_blocks(const CodestreamInfo& ci, const std::vector<DecBlock>& 
 
 // decode/mod.rs:56-129 up to the Tier-1 decode: every host-side step of
 // decode_jp2, with the device path's own limits.
-void parse_for_decode(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
+void parse_decode_header(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamInfo& ci) {
     const uint8_t* cs;
     size_t cslen;
     parse_jp2(bytes, len, jp, cs, cslen);
@@ -970,6 +989,9 @@ void parse_for_decode(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamI
     if (ci.width > 65535 || ci.height > 65535)
         unsupported("image dimensions above 65535 are not supported by the device path");
     if (ci.cbw > 64 || ci.cbh > 64) unsupported("code-blocks larger than 64x64 are not supported by the device path");
+}
+void parse_for_decode(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
+    parse_decode_header(bytes, len, jp, ci);
     parse_packets(ci, blocks);
     check_decode_blocks(ci, blocks);
 }

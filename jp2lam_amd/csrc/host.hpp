@@ -142,8 +142,35 @@ void parse_codestream(const uint8_t* b, size_t n, CodestreamInfo& ci);
 // decode/t2.rs:75-256: walk the LRCP packets; fills blocks (component-major
 // band order, raster within band).
 void parse_packets(const CodestreamInfo& ci, std::vector<DecBlock>& blocks);
+// The same walk in steps: the constructor lays out every band's blocks,
+// run(rmax) parses the packets of the resolutions up to rmax not parsed yet
+// (one layer, so LRCP order is resolution-major), finish() checks that the
+// payload is used up.  parse_packets is run(levels) then finish().
+class PacketParser {
+public:
+    PacketParser(const CodestreamInfo& ci, std::vector<DecBlock>& blocks);
+    ~PacketParser();
+    void run(int rmax);
+    void finish();
+    // the header and block list moved (their contents unchanged)
+    void rebind(const CodestreamInfo& ci, std::vector<DecBlock>& blocks) {
+        ci_ = &ci;
+        blocks_ = &blocks;
+    }
+
+private:
+    struct Band;
+    const CodestreamInfo* ci_;
+    std::vector<DecBlock>* blocks_;
+    std::vector<Band> bands_;
+    int layer_ = 0, res_ = 0;
+    size_t pos_ = 0;
+};
 // decode/t1.rs:96-127 per-block checks (zero bit-planes, pass counts).
 void check_decode_blocks(const CodestreamInfo& ci, const std::vector<DecBlock>& blocks);
+// The container and codestream part of parse_for_decode: JP2 boxes,
+// markers and the header checks, no packets.
+void parse_decode_header(const uint8_t* bytes, size_t len, Jp2Info& jp, CodestreamInfo& ci);
 // Every host-side step of decode_jp2 before Tier-1 (decode/mod.rs:56-129):
 // JP2 boxes, codestream markers, header consistency, packets, block checks.
 // Throws Error(DECODE_FAILED / UNSUPPORTED_FEATURE) on malformed input.
