@@ -569,3 +569,35 @@ def test_pixel_kernels_unaligned_device_buffers(gpu, ora, lib, w, h, ch, q):
     finally:
         lib.jp2lam_device_free(d_in)
         lib.jp2lam_device_free(d_out)
+
+
+def _truncate_tile(data: bytes, cut: int) -> bytes:
+    """The JP2 stream with the last `cut` bytes of its tile payload removed:
+    Psot and the jp2c box length shrink with it, EOC follows."""
+    b = bytearray(data)
+    box = b.find(b"jp2c") - 4
+    sot = b.find(b"\xff\x90", box + 8)
+    psot = int.from_bytes(b[sot + 6:sot + 10], "big")
+    end = sot + psot  # EOC follows the tile part
+    assert b[end:end + 2] == b"\xff\xd9"
+    out = b[:end - cut] + b"\xff\xd9"
+    out[sot + 6:sot + 10] = (psot - cut).to_bytes(4, "big")
+    lbox = int.from_bytes(out[box:box + 4], "big")
+    if lbox:
+        out[box:box + 4] = (lbox - cut).to_bytes(4, "big")
+    return bytes(out)
+
+
+@pytest.mark.parametrize("cut", [1, 9, 400, 30000])
+def test_decode_split_errors_like_the_reference(gpu, ora, cut):
+    """A single image is decoded in two parts (coarse resolutions first, the
+    two finest while the first part runs on the device).  Streams cut short
+    in the finest resolutions' packets (small cuts) or in the coarse ones
+    (the large cut) fail with the oracle's error, and the untouched stream
+    decodes to the oracle's pixels."""
+    px = synth.document_page(600, 520, seed=31)
+    data = j.encode_u8(px, 85)
+    assert data == ora.encode(px, 85)
+    assert expect_same_decode(ora, data) is not None
+    bad = _truncate_tile(data, cut)
+    assert expect_same_decode(ora, bad) is None
