@@ -55,7 +55,10 @@ def oracle_forward(ora, px, q):
     return np.stack(out), levels
 
 
-SIZES = [(1, 1), (2, 3), (7, 5), (17, 19), (64, 64), (65, 63), (130, 200), (512, 512)]
+SIZES = [(1, 1), (2, 3), (7, 5), (17, 19), (64, 64), (65, 63), (130, 200), (512, 512),
+         # one- and few-sample rows or columns, and tile edges inside a
+         # lifting window
+         (1, 300), (300, 1), (3, 129), (129, 3), (250, 66)]
 
 
 @pytest.mark.parametrize("w,h", SIZES)
@@ -87,7 +90,8 @@ def test_forward_97_error_vs_fp64(gpu, ora):
     assert rel < 1e-5, rel
 
 
-@pytest.mark.parametrize("w,h,lv", [(1, 1, 0), (2, 2, 1), (9, 4, 2), (64, 64, 5), (131, 97, 5), (300, 255, 6)])
+@pytest.mark.parametrize("w,h,lv", [(1, 1, 0), (2, 2, 1), (9, 4, 2), (64, 64, 5), (131, 97, 5), (300, 255, 6),
+                                    (1, 200, 0), (3, 130, 1), (2, 131, 1), (250, 66, 2)])
 @pytest.mark.parametrize("irrev", [0, 1])
 def test_inverse_dwt_bit_exact(gpu, ora, w, h, lv, irrev):
     rng = np.random.default_rng(w * h + lv)
