@@ -1342,6 +1342,10 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
             outs[im.idx].status = e.status;
             outs[im.idx].message = e.what();
             return outs;
+        } catch (...) {
+            // the first part still runs on the arena: let it finish first
+            (void)hipStreamSynchronize(st);
+            throw;
         }
         std::vector<DecodeJob> rest;
         rest.reserve(nj_all - nj);
