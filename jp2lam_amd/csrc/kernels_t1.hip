@@ -846,9 +846,15 @@ __device__ __forceinline__ int load_block(const BlockJob& J, uint32_t (&mag)[64]
 // its most loaded SIMD (a block with many bit-planes runs at raised issue
 // priority and starves the wave beside it).  So blocks are handed out in
 // descending estimated cost: the first wave to start on a SIMD takes the next
-// heavy block, any later wave there the next light one.  sched holds
-// cost[n] | order[n] | heavy count, light count | arrivals per SIMD.
+// heavy block, any later wave there the next light one.  When a launch's
+// waves do not divide evenly, some SIMDs start with three waves; a heavy
+// block beside two others ends late (measured on a 2550x3300 page: the
+// launch ended 0.3 ms after its heaviest block), so a SIMD's first wave
+// waits until the launch's waves have arrived (at most kPickWaitTicks) and
+// takes a light block too if its SIMD runs three.  sched holds
+// cost[n] | order[n] | heavy count, light count, arrivals | arrivals per SIMD.
 constexpr uint32_t kSimdSlots = 1u << 14;  // xcc(4) se(3) sh(1) cu(4) simd(2)
+constexpr uint64_t kPickWaitTicks = 2500;   // 25 us of the 100 MHz real-time clock
 
 // Estimated cost of a block in MQ decisions: fitted on document and photo
 // pages (correlation 0.98 with the decisions coded), plus about 470
@@ -879,7 +885,7 @@ __global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t* __restr
     const int tid = threadIdx.x;
     if (tid < 256) hist[tid] = 0;
     uint32_t* ctrl = sched + 2 * (size_t)n;
-    for (uint32_t i = tid; i < 2u + kSimdSlots; i += 1024u) ctrl[i] = 0;
+    for (uint32_t i = tid; i < 3u + kSimdSlots; i += 1024u) ctrl[i] = 0;
     __syncthreads();
     for (int b = tid; b < n; b += 1024) atomicAdd(&hist[255u - min(255u, sched[b] >> 9)], 1u);
     __syncthreads();
@@ -896,7 +902,10 @@ __global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t* __restr
 }
 // The block this wave codes.  The n waves of the launch take n distinct
 // positions of the order (heavy ones from the front, light ones from the
-// back), wherever they are placed, so every block is coded exactly once.
+// back), wherever they are placed and however the wait ends, so every block
+// is coded exactly once.  The wait is bounded by the clock: waves beyond the
+// resident ones start only as others end, and then the total never reaches n
+// within it.
 __device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched, uint32_t& pos) {
     uint32_t b = 0, at = 0;
     if (lid() == 0) {
@@ -905,9 +914,18 @@ __device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched, u
         const uint32_t simd = ((xcc & 15u) << 10) | (((hw >> 13) & 7u) << 7) | (((hw >> 12) & 1u) << 6) |
                               (((hw >> 8) & 15u) << 2) | ((hw >> 4) & 3u);
         uint32_t* ctrl = sched + 2 * (size_t)n;
-        const bool first = atomicAdd(&ctrl[2 + simd], 1u) == 0u;
-        const uint32_t k = atomicAdd(&ctrl[first ? 0 : 1], 1u);
-        at = first ? k : n - 1u - k;
+        uint32_t* const here = &ctrl[3 + simd];
+        bool heavy = atomicAdd(here, 1u) == 0u;
+        atomicAdd(&ctrl[2], 1u);
+        if (heavy) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            while (__hip_atomic_load(&ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n &&
+                   __builtin_amdgcn_s_memrealtime() - t0 < kPickWaitTicks)
+                __builtin_amdgcn_s_sleep(4);
+            heavy = __hip_atomic_load(here, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 3u;
+        }
+        const uint32_t k = atomicAdd(&ctrl[heavy ? 0 : 1], 1u);
+        at = heavy ? k : n - 1u - k;
         b = sched[n + at];
     }
     pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
@@ -936,7 +954,7 @@ constexpr uint32_t kPrePlanes = 32;
 constexpr uint32_t kPlaneWords = (kSymCap + 3u * 64u) / 4u;
 constexpr uint32_t kPreSlotWords = 2u * kMaxPasses + kPrePlanes * kPlaneWords;
 __host__ __device__ __forceinline__ size_t pre_base(uint32_t n) {
-    return (2 * (size_t)n + 2 + kSimdSlots + 63) & ~(size_t)63;
+    return (2 * (size_t)n + 3 + kSimdSlots + 63) & ~(size_t)63;
 }
 __device__ __forceinline__ bool pre_formed(const uint32_t* sched, uint32_t n, uint32_t pos) {
     if (pos >= kPre || pos >= n) return false;

@@ -11,7 +11,11 @@
 //   pick   the shipped hand-out (t1_pick_block in kernels_t1.hip): a wave
 //          counts itself on its SIMD; the first wave on a SIMD takes the next
 //          position from the front of the cost order, any later one the next
-//          from the back.  One block per wave, n waves.  No wave waits.
+//          from the back.  The first wave decides after a wait bounded by the
+//          clock, taking from the back too if its SIMD has three arrivals by
+//          then; the explorer lets that decision fall anywhere between the
+//          other waves' steps, which covers every way the wait can end.  One
+//          block per wave, n waves.  No wave waits on another.
 //   claim  persistent waves: each claims positions from one counter until
 //          none are left.  No wave waits.
 //   mark   the exclusive-SIMD form: the first wave on a SIMD claims a
@@ -99,10 +103,13 @@ struct Model {
         case PICK:
             if (v.pc == 0) {  // atomicAdd on the SIMD's arrival counter
                 v.first = s.simd_count[v.simd]++ == 0;
-                v.pc = 1;
-            } else if (v.pc == 1) {  // atomicAdd on the front or back counter
-                v.pos = v.first ? s.front++ : blocks - 1 - s.back++;
+                v.pc = v.first ? 1 : 2;
+            } else if (v.pc == 1) {  // first wave, after its bounded wait: heavy
+                v.first = s.simd_count[v.simd] < 3;  // unless its SIMD runs three
                 v.pc = 2;
+            } else if (v.pc == 2) {  // atomicAdd on the front or back counter
+                v.pos = v.first ? s.front++ : blocks - 1 - s.back++;
+                v.pc = 3;
             } else {  // code the block, exit
                 if (v.pos >= 0 && v.pos < blocks) s.coded[v.pos]++;
                 v.done = true;

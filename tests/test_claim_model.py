@@ -4,8 +4,9 @@ Every interleaving of the waves' atomic steps is explored, with in-order
 dispatch into a bounded number of resident slots:
 
 - the shipped hand-out (t1_pick_block: per-SIMD arrival counter, heavy
-  positions from the front for a SIMD's first wave, light ones from the back
-  for later waves) codes every block exactly once and never blocks, for every
+  positions from the front for a SIMD's first wave unless three waves share
+  its SIMD when its bounded wait ends, light ones from the back for every
+  other wave) codes every block exactly once and never blocks, for every
   resident count, including fewer slots than waves;
 - a persistent claim loop without waits does the same;
 - the exclusive-SIMD form, in which a later wave on a SIMD waits for the
@@ -38,7 +39,8 @@ def _run(exe, proto, waves, blocks, simds, resident):
     return r.returncode, r.stdout.strip()
 
 
-@pytest.mark.parametrize("cfg", [(4, 4, 2, 4), (4, 4, 2, 2), (4, 4, 2, 1), (6, 6, 2, 3), (6, 6, 3, 2), (8, 8, 4, 3)])
+@pytest.mark.parametrize("cfg", [(4, 4, 2, 4), (4, 4, 2, 2), (4, 4, 2, 1), (6, 6, 2, 3), (6, 6, 3, 2), (8, 8, 4, 3),
+                                 (6, 6, 2, 6), (7, 7, 2, 7), (7, 7, 3, 5)])
 def test_shipped_hand_out_codes_every_block_once(model, cfg):
     rc, out = _run(model, "pick", *cfg)
     assert rc == 0 and out.startswith("ok"), out
