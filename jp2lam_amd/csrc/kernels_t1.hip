@@ -17,6 +17,9 @@
 // byte-position snapshots used as pass lengths (t1.rs:583-600).
 #include "common.hpp"
 
+#include <algorithm>
+#include <mutex>
+
 namespace j2l {
 
 namespace {
@@ -620,6 +623,58 @@ __device__ __forceinline__ void mqe_symbol(MqE& m, uint32_t ctx, uint32_t d31) {
         : [d] "s"(d31), [tab] "v"(m.tab), [f16] "s"(0x60010u)
         : "scc");
 }
+// mqe_symbol with the context states in s[80:98].
+__device__ __forceinline__ void mqe_symbol_s(MqE& m, uint32_t ctx, uint32_t d31) {
+    uint32_t e, q, u, t;
+    asm volatile(
+        "s_nop 0\n\t"                 // the M0 wait state
+        "s_movrels_b32 %[e], s80\n\t"
+        "s_and_b32 %[q], %[e], 0xffff\n\t"
+        "s_sub_u32 %[a], %[a], %[q]\n\t"
+        "s_xor_b32 %[u], %[e], %[d]\n\t"
+        "s_lshl_b32 %[t], %[a], 16\n\t"
+        "s_andn2_b32 %[t], %[t], %[u]\n\t"
+        "s_cmp_lt_i32 %[t], 0\n\t"
+        "s_cbranch_scc0 7f\n\t"       // renormalising path: the one taken branch
+        "s_add_u32 %[c], %[c], %[q]\n\t"
+        "s_branch 9f\n"
+        "1:\n\t"                      // t >= ct: shift by ct and move a byte out
+        MQE_BYTEOUT("2", "3", "5", "6", "8f")
+        "7:\n\t"
+        // renormalising decision, MPS or LPS, branch-free: conditional
+        // exchange to_qe = (a < qe) XOR lps; MPS: a = qe if a < qe else
+        // c += qe; LPS: c += qe if a < qe else a = qe; state NMPS / NLPS,
+        // MPS ^= SWITCH on LPS
+        // (a, qe < 2^16, so bit 31 of a - qe is a < qe)
+        "s_sub_u32 %[t], %[a], %[q]\n\t"
+        "s_xor_b32 %[t], %[t], %[u]\n\t"
+        "s_cmp_lt_i32 %[t], 0\n\t"
+        "s_cselect_b32 %[a], %[q], %[a]\n\t"
+        "s_cselect_b32 %[t], 0, %[q]\n\t"
+        "s_add_u32 %[c], %[c], %[t]\n\t"
+        "s_cmp_lt_i32 %[u], 0\n\t"
+        "s_cselect_b32 %[t], 0x60016, %[f16]\n\t"  // NLPS / NMPS field: width 6 at 22 / 16
+        "s_bfe_u32 %[t], %[e], %[t]\n\t"
+        "v_readlane_b32 %[t], %[tab], %[t]\n\t"
+        "s_and_b32 %[q], %[e], 0x80000000\n\t"
+        "s_xor_b32 %[t], %[t], %[q]\n\t"             // bit 31: MPS, flipped by a switching LPS
+        "s_movreld_b32 s80, %[t]\n\t"
+        "s_flbit_i32_b32 %[t], %[a]\n\t"
+        "s_sub_u32 %[t], %[t], 16\n"
+        // renormalise by t bits, a byte out whenever ct reaches zero
+        "8:\n\t"
+        "s_cmp_lt_u32 %[t], %[ct]\n\t"
+        "s_cbranch_scc0 1b\n\t"
+        "s_lshl_b32 %[a], %[a], %[t]\n\t"
+        "s_lshl_b32 %[c], %[c], %[t]\n\t"
+        "s_sub_u32 %[ct], %[ct], %[t]\n"
+        "9:"
+        : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
+          [win] "+v"(m.win), [m0] "+{m0}"(ctx), [e] "=&s"(e), [q] "=&s"(q),
+          [u] "=&s"(u), [t] "=&s"(t)
+        : [d] "s"(d31), [tab] "v"(m.tab), [f16] "s"(0x60010u)
+        : "scc");
+}
 // Coder registers at the start of a block (mq/mod.rs:183-200).  Their
 // initial values come from one asm that writes SGPRs: a constant the
 // compiler materialises with v_mov (0 is shared with the ring, a VGPR) would
@@ -646,9 +701,12 @@ __device__ __forceinline__ void mqe_init(MqE& m, uint32_t mq, uint8_t* out, uint
 // MPS case that keeps a >= 0x8000 is one taken branch to the next decision;
 // the renormalising path falls through (and takes qe back from c when it
 // exchanges), with the byte-out out of line after the group.
-#define MQE_G_SYM(ID, X, OFF, DSH, DREG)                                           \
+#define MQE_G_SYM(ID, X, OFF, DSH, DREG) MQE_G_SYM_AT(ID, X, OFF, DSH, DREG, MQE_V_READ, MQE_V_WRITE)
+#define MQE_V_READ "v_readlane_b32 %[e], %[cx], m0\n\t"
+#define MQE_V_WRITE "v_writelane_b32 %[cx], %[t], m0\n\t"
+#define MQE_G_SYM_AT(ID, X, OFF, DSH, DREG, SREAD, SWRITE)                         \
     "s_bfe_u32 m0, %[" X "], " OFF "\n\t" DSH                                     \
-    "v_readlane_b32 %[e], %[cx], m0\n\t"                                         \
+    SREAD                                                                        \
     "s_and_b32 %[q], %[e], 0xffff\n\t"                                           \
     "s_sub_u32 %[a], %[a], %[q]\n\t"                                             \
     "s_xor_b32 %[u], %[e], " DREG "\n\t"                                         \
@@ -669,7 +727,7 @@ __device__ __forceinline__ void mqe_init(MqE& m, uint32_t mq, uint8_t* out, uint
     "v_readlane_b32 %[t], %[tab], %[t]\n\t"                                      \
     "s_and_b32 %[q], %[e], 0x80000000\n\t"                                       \
     "s_xor_b32 %[t], %[t], %[q]\n\t"                                             \
-    "v_writelane_b32 %[cx], %[t], m0\n\t"                                        \
+    SWRITE                                                                       \
     "s_flbit_i32_b32 %[t], %[a]\n\t"                                             \
     "s_sub_u32 %[t], %[t], 16\n"                                                  \
     "1" ID "2:\n\t"                                                              \
@@ -685,6 +743,18 @@ __device__ __forceinline__ void mqe_init(MqE& m, uint32_t mq, uint8_t* out, uint
     MQE_G_SYM(I1, X, "0x50008", "s_lshl_b32 %[d], %[" X "], 16\n\t", "%[d]")     \
     MQE_G_SYM(I2, X, "0x50010", "s_lshl_b32 %[d], %[" X "], 8\n\t", "%[d]")      \
     MQE_G_SYM(I3, X, "0x50018", "", "%[" X "]")
+
+// The same with the context states in s[80:98] (the decoder's block, see
+// states_init): s_movrels / s_movreld instead of the lane read and write.
+// An SALU write of M0 needs one wait state before s_movrels; the shift
+// forming the decision fills it for the first three bytes of a word.
+#define MQE_S_READ "s_movrels_b32 %[e], s80\n\t"
+#define MQE_S_WRITE "s_movreld_b32 s80, %[t]\n\t"
+#define MQE_GS_WORD(I0, I1, I2, I3, X)                                                                   \
+    MQE_G_SYM_AT(I0, X, "0x50000", "s_lshl_b32 %[d], %[" X "], 24\n\t", "%[d]", MQE_S_READ, MQE_S_WRITE) \
+    MQE_G_SYM_AT(I1, X, "0x50008", "s_lshl_b32 %[d], %[" X "], 16\n\t", "%[d]", MQE_S_READ, MQE_S_WRITE) \
+    MQE_G_SYM_AT(I2, X, "0x50010", "s_lshl_b32 %[d], %[" X "], 8\n\t", "%[d]", MQE_S_READ, MQE_S_WRITE)  \
+    MQE_G_SYM_AT(I3, X, "0x50018", "s_nop 0\n\t", "%[" X "]", MQE_S_READ, MQE_S_WRITE)
 
 // 16 decisions, the symbol bytes of words x0..x3 in order (at most 32
 // bytes move out).
@@ -703,6 +773,26 @@ __device__ __forceinline__ void mqe_group16(MqE& m, uint32_t x0, uint32_t x1, ui
         "9:"
         : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
           [cx] "+v"(m.cx), [win] "+v"(m.win), [e] "=&s"(e), [q] "=&s"(q), [u] "=&s"(u),
+          [t] "=&s"(t), [d] "=&s"(d)
+        : [x0] "s"(x0), [x1] "s"(x1), [x2] "s"(x2), [x3] "s"(x3), [tab] "v"(m.tab), [f16] "s"(0x60010u)
+        : "scc", "m0");
+}
+// mqe_group16 with the context states in s[80:98].
+__device__ __forceinline__ void mqe_group16s(MqE& m, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+    uint32_t e, q, u, t, d;
+    asm volatile(
+        MQE_GS_WORD("00", "01", "02", "03", "x0")
+        MQE_GS_WORD("04", "05", "06", "07", "x1")
+        MQE_GS_WORD("08", "09", "10", "11", "x2")
+        MQE_GS_WORD("12", "13", "14", "15", "x3")
+        "s_branch 9f\n"
+        MQE_G_BO("00") MQE_G_BO("01") MQE_G_BO("02") MQE_G_BO("03")
+        MQE_G_BO("04") MQE_G_BO("05") MQE_G_BO("06") MQE_G_BO("07")
+        MQE_G_BO("08") MQE_G_BO("09") MQE_G_BO("10") MQE_G_BO("11")
+        MQE_G_BO("12") MQE_G_BO("13") MQE_G_BO("14") MQE_G_BO("15")
+        "9:"
+        : [a] "+s"(m.a), [c] "+s"(m.c), [ct] "+s"(m.ct), [pos] "+s"(m.pos), [last] "+s"(m.last),
+          [win] "+v"(m.win), [e] "=&s"(e), [q] "=&s"(q), [u] "=&s"(u),
           [t] "=&s"(t), [d] "=&s"(d)
         : [x0] "s"(x0), [x1] "s"(x1), [x2] "s"(x2), [x3] "s"(x3), [tab] "v"(m.tab), [f16] "s"(0x60010u)
         : "scc", "m0");
@@ -741,6 +831,29 @@ __device__ __forceinline__ void mqe_chunk(MqE& m, uint32_t sv, uint32_t cnt) {
     if (q < cnt) {
         uint32_t w = rl(sv, q >> 2);
         for (; q < cnt; q++, w >>= 8) mqe_symbol(m, w & 31u, w << 24);
+        mqe_spill(m);
+    }
+}
+// mqe_chunk with the context states in s[80:98].
+__device__ __forceinline__ void mqe_chunk_s(MqE& m, uint32_t sv, uint32_t cnt) {
+    uint32_t q = 0;
+    // 16 symbols move at most 32 bytes out, so one spill check per 16
+    // keeps the ring from reusing an unstored byte (at most 64 pending)
+    for (; q + 16u <= cnt; q += 16u) {
+        mqe_group16s(m, rl(sv, q >> 2), rl(sv, (q >> 2) + 1u), rl(sv, (q >> 2) + 2u), rl(sv, (q >> 2) + 3u));
+        mqe_spill(m);
+    }
+    for (; q + 4u <= cnt; q += 4u) {
+        const uint32_t w = rl(sv, q >> 2);
+        mqe_symbol_s(m, w & 31u, w << 24);
+        mqe_symbol_s(m, (w >> 8) & 31u, w << 16);
+        mqe_symbol_s(m, (w >> 16) & 31u, w << 8);
+        mqe_symbol_s(m, (w >> 24) & 31u, w);
+        mqe_spill(m);
+    }
+    if (q < cnt) {
+        uint32_t w = rl(sv, q >> 2);
+        for (; q < cnt; q++, w >>= 8) mqe_symbol_s(m, w & 31u, w << 24);
         mqe_spill(m);
     }
 }
@@ -846,12 +959,20 @@ __device__ __forceinline__ int load_block(const BlockJob& J, uint32_t (&mag)[64]
 // its most loaded SIMD (a block with many bit-planes runs at raised issue
 // priority and starves the wave beside it).  So blocks are handed out in
 // descending estimated cost: the first wave to start on a SIMD takes the next
-// heavy block, any later wave there the next light one.  When a launch's
-// waves do not divide evenly, some SIMDs start with three waves; a heavy
-// block beside two others ends late (measured on a 2550x3300 page: the
-// launch ended 0.3 ms after its heaviest block), so a SIMD's first wave
-// waits until the launch's waves have arrived (at most kPickWaitTicks) and
-// takes a light block too if its SIMD runs three.  sched holds
+// heavy block, any later wave there a light one.  When every wave of a
+// launch is resident and the waves do not divide evenly, some SIMDs start
+// with three, and a heavy block beside two others ends late (measured on a
+// 2550x3300 page: the launch ended 0.3 ms after its heaviest block).  For
+// such launches (`three`) a SIMD's first wave waits until the launch's waves
+// have arrived (at most kPickWaitTicks) and takes a light block too if its
+// SIMD runs three.  The order is laid out for the two ends: the `split`
+// heaviest blocks in descending cost from the front, where the heavy takers
+// read, and the rest in ascending cost from there to the back, where the
+// light takers read from the end inward.  So the light takers get the
+// heavier of the light blocks first and the lightest last: the waves that
+// arrive last (third waves and the first waves of SIMDs with three, which
+// decide after their wait) get the lightest, and a launch whose waves are
+// not all resident hands out its light end longest first.  sched holds
 // cost[n] | order[n] | heavy count, light count, arrivals | arrivals per SIMD.
 constexpr uint32_t kSimdSlots = 1u << 14;  // xcc(4) se(3) sh(1) cu(4) simd(2)
 constexpr uint64_t kPickWaitTicks = 2500;   // 25 us of the 100 MHz real-time clock
@@ -878,9 +999,10 @@ __global__ __launch_bounds__(64) void t1_cost_kernel(const BlockJob* __restrict_
     z = wave_sum(z);
     if (lid() == 0) sched[blockIdx.x] = mbp ? (9u * bits + 4u * z + 12u * R.nnz) / 8u + 470u * (uint32_t)mbp : 0u;
 }
-// Counting sort of the blocks by descending cost (512-decision buckets) and
-// reset of the hand-out counters.  One workgroup.
-__global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t* __restrict__ sched) {
+// Counting sort of the blocks by descending cost (512-decision buckets), the
+// ranks from `split` on placed back to front, and reset of the hand-out
+// counters.  One workgroup.
+__global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t split, uint32_t* __restrict__ sched) {
     __shared__ uint32_t hist[256];
     const int tid = threadIdx.x;
     if (tid < 256) hist[tid] = 0;
@@ -898,7 +1020,10 @@ __global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t* __restr
         }
     }
     __syncthreads();
-    for (int b = tid; b < n; b += 1024) sched[n + atomicAdd(&hist[255u - min(255u, sched[b] >> 9)], 1u)] = (uint32_t)b;
+    for (int b = tid; b < n; b += 1024) {
+        const uint32_t k = atomicAdd(&hist[255u - min(255u, sched[b] >> 9)], 1u);
+        sched[n + (k < split ? k : split + ((uint32_t)n - 1u - k))] = (uint32_t)b;
+    }
 }
 // The block this wave codes.  The n waves of the launch take n distinct
 // positions of the order (heavy ones from the front, light ones from the
@@ -906,7 +1031,7 @@ __global__ __launch_bounds__(1024) void t1_order_kernel(int n, uint32_t* __restr
 // is coded exactly once.  The wait is bounded by the clock: waves beyond the
 // resident ones start only as others end, and then the total never reaches n
 // within it.
-__device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched, uint32_t& pos) {
+__device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t three, uint32_t* sched, uint32_t& pos) {
     uint32_t b = 0, at = 0;
     if (lid() == 0) {
         uint32_t hw, xcc;
@@ -917,7 +1042,7 @@ __device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched, u
         uint32_t* const here = &ctrl[3 + simd];
         bool heavy = atomicAdd(here, 1u) == 0u;
         atomicAdd(&ctrl[2], 1u);
-        if (heavy) {
+        if (heavy && three) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
             while (__hip_atomic_load(&ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n &&
                    __builtin_amdgcn_s_memrealtime() - t0 < kPickWaitTicks)
@@ -949,7 +1074,7 @@ __device__ __forceinline__ uint32_t t1_pick_block(uint32_t n, uint32_t* sched, u
 // kPlaneWords.  In a plane region the passes follow each other, each
 // rounded up to 64 bytes (the coder's read unit); a plane has at most 10
 // symbols per stripe-column, 10240 in all.
-constexpr uint32_t kPre = 8;
+constexpr uint32_t kPre = 32;
 constexpr uint32_t kPrePlanes = 32;
 constexpr uint32_t kPlaneWords = (kSymCap + 3u * 64u) / 4u;
 constexpr uint32_t kPreSlotWords = 2u * kMaxPasses + kPrePlanes * kPlaneWords;
@@ -1037,15 +1162,16 @@ __device__ __forceinline__ W16 load16(const uint32_t* p) {
 // t1_pick_block).  Outputs the terminated segment into out + job.out_off,
 // the per-pass byte snapshots and newly-significant counts, and the block
 // statistics.
-__global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap), amdgpu_waves_per_eu(1, 8))) void t1_encode_kernel(const BlockJob* __restrict__ jobs, const uint32_t* __restrict__ tabs,
                                                        uint8_t* __restrict__ out, BlockResult* __restrict__ results,
                                                        uint32_t* __restrict__ snaps, uint16_t* __restrict__ newsigs,
-                                                       uint32_t* __restrict__ sched, const uint32_t* __restrict__ pre) {
-    __shared__ uint32_t symw[kSymCap / 4];
+                                                       uint32_t* __restrict__ sched, const uint32_t* __restrict__ pre,
+                                                       uint32_t three) {
+    extern __shared__ uint32_t symw[];
     uint8_t* symb = reinterpret_cast<uint8_t*>(symw);
     const int lane = lid();
     uint32_t pos;
-    const uint32_t blk = t1_pick_block(gridDim.x, sched, pos);
+    const uint32_t blk = t1_pick_block(gridDim.x, three, sched, pos);
     const BlockJob J = jobs[blk];
     const int w = J.w, h = J.h;
     uint32_t mag[64];
@@ -1082,6 +1208,11 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
     if (pre_formed(sched, gridDim.x, pos)) {
         // Code the pre-formed passes: 64 symbols per step from SGPRs, the
         // next 64 (or the next pass's first) loaded while these are coded.
+        // The context states move to s[80:98] for the block: the wave coding
+        // the heaviest block sets the launch's length, and the scalar reads
+        // and writes shorten its decisions (the light blocks keep the states
+        // in a VGPR, which leaves the scalar unit to the waves beside them).
+        states_init(m.cx);
         const uint32_t* slotp = pre + (size_t)pos * kPreSlotWords;
         const uint32_t* data = slotp + 2u * kMaxPasses;
         const uint32_t hs0 = slotp[lane], hn0 = slotp[kMaxPasses + lane];
@@ -1104,13 +1235,13 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
             for (; b + 64u <= cnt; b += 64u) {
                 const uint32_t nxt = b + 64u < cnt ? at + 16u : after;
                 const W16 nx = load16(data + nxt);
-                mqe_group16(m, cur.w[0], cur.w[1], cur.w[2], cur.w[3]);
+                mqe_group16s(m, cur.w[0], cur.w[1], cur.w[2], cur.w[3]);
                 mqe_spill(m);
-                mqe_group16(m, cur.w[4], cur.w[5], cur.w[6], cur.w[7]);
+                mqe_group16s(m, cur.w[4], cur.w[5], cur.w[6], cur.w[7]);
                 mqe_spill(m);
-                mqe_group16(m, cur.w[8], cur.w[9], cur.w[10], cur.w[11]);
+                mqe_group16s(m, cur.w[8], cur.w[9], cur.w[10], cur.w[11]);
                 mqe_spill(m);
-                mqe_group16(m, cur.w[12], cur.w[13], cur.w[14], cur.w[15]);
+                mqe_group16s(m, cur.w[12], cur.w[13], cur.w[14], cur.w[15]);
                 mqe_spill(m);
                 cur = nx;
                 at = nxt;
@@ -1121,7 +1252,7 @@ __global__ __launch_bounds__(64) void t1_encode_kernel(const BlockJob* __restric
                 uint32_t sv = 0;
 #pragma unroll
                 for (int i = 0; i < 16; i++) sv = wlane(sv, cur.w[i], (uint32_t)i);
-                mqe_chunk(m, sv, cnt - b);
+                mqe_chunk_s(m, sv, cnt - b);
                 cur = nx;
                 at = after;
             }
@@ -2024,13 +2155,52 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_num_sgpr(kStateSgprCap)))
 }
 
 size_t t1_sched_words(int n) { return pre_base((uint32_t)n) + kPre * (size_t)kPreSlotWords; }
+
+namespace {
+// SIMDs of the current device and encode waves resident on it at once
+// (zeros if the queries fail), queried once per device.
+struct EncodeShape {
+    uint32_t simds = 0, resident = 0;
+};
+EncodeShape encode_shape() {
+    static std::mutex mu;
+    static EncodeShape cache[64];
+    static bool known[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return EncodeShape{};
+    std::lock_guard<std::mutex> g(mu);
+    if (!known[dev]) {
+        int cus = 0, per_cu = 0;
+        EncodeShape e;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, t1_encode_kernel, 64, kSymCap) == hipSuccess &&
+            cus > 0 && per_cu > 0) {
+            e.simds = 4u * (uint32_t)cus;
+            e.resident = (uint32_t)cus * (uint32_t)per_cu;
+        }
+        cache[dev] = e;
+        known[dev] = true;
+    }
+    return cache[dev];
+}
+}  // namespace
+
 void launch_t1_encode(const BlockJob* jobs, int n, const uint32_t* tabs, uint8_t* out, BlockResult* res, uint32_t* snaps,
                       uint16_t* newsigs, uint32_t* sched, hipStream_t st) {
     if (n <= 0) return;
+    // Heavy takers: one per SIMD, none on SIMDs that run three waves when
+    // every wave is resident (all SIMDs hold two, 3 * simds - n hold only
+    // two).  The rule stays off where fewer than kPre SIMDs would take heavy
+    // blocks, so the pre-formed positions are always heavy takers' ones.
+    const EncodeShape e = encode_shape();
+    const uint32_t un = (uint32_t)n;
+    const bool three = e.simds && un > 2u * e.simds && un <= e.resident && un + kPre <= 3u * e.simds;
+    const uint32_t split = !e.simds ? un : three ? 3u * e.simds - un : std::min(un, e.simds);
     t1_cost_kernel<<<n, 64, 0, st>>>(jobs, sched);
-    t1_order_kernel<<<1, 1024, 0, st>>>(n, sched);
-    t1_preform_kernel<<<kPre * kPrePlanes, 64, 0, st>>>(jobs, tabs, (uint32_t)n, sched);
-    t1_encode_kernel<<<n, 64, 0, st>>>(jobs, tabs, out, res, snaps, newsigs, sched, sched + pre_base((uint32_t)n));
+    t1_order_kernel<<<1, 1024, 0, st>>>(n, split, sched);
+    t1_preform_kernel<<<kPre * kPrePlanes, 64, 0, st>>>(jobs, tabs, un, sched);
+    t1_encode_kernel<<<n, 64, kSymCap, st>>>(jobs, tabs, out, res, snaps, newsigs, sched, sched + pre_base(un),
+                                             three ? 1u : 0u);
 }
 void launch_t1_form(const BlockJob* jobs, int n, const uint32_t* tabs, BlockResult* res, uint32_t* pend, uint16_t* newsigs,
                     uint32_t* soff16, uint8_t* pool, unsigned long long* pool_used, unsigned long long pool_cap,

@@ -12,10 +12,11 @@
 //          counts itself on its SIMD; the first wave on a SIMD takes the next
 //          position from the front of the cost order, any later one the next
 //          from the back.  The first wave decides after a wait bounded by the
-//          clock, taking from the back too if its SIMD has three arrivals by
-//          then; the explorer lets that decision fall anywhere between the
-//          other waves' steps, which covers every way the wait can end.  One
-//          block per wave, n waves.  No wave waits on another.
+//          clock (heavy unless the launch's rule is on and its SIMD has three
+//          arrivals by then); the explorer lets the decision fall anywhere
+//          between the other waves' steps and takes both outcomes, so every
+//          way the wait can end and either setting of the rule is covered.
+//          One block per wave, n waves.  No wave waits on another.
 //   claim  persistent waves: each claims positions from one counter until
 //          none are left.  No wave waits.
 //   mark   the exclusive-SIMD form: the first wave on a SIMD claims a
@@ -104,9 +105,8 @@ struct Model {
             if (v.pc == 0) {  // atomicAdd on the SIMD's arrival counter
                 v.first = s.simd_count[v.simd]++ == 0;
                 v.pc = v.first ? 1 : 2;
-            } else if (v.pc == 1) {  // first wave, after its bounded wait: heavy
-                v.first = s.simd_count[v.simd] < 3;  // unless its SIMD runs three
-                v.pc = 2;
+            } else if (v.pc == 1) {  // first wave, after its bounded wait
+                return false;             // heavy or light: both explored (explore)
             } else if (v.pc == 2) {  // atomicAdd on the front or back counter
                 v.pos = v.first ? s.front++ : blocks - 1 - s.back++;
                 v.pc = 3;
@@ -200,6 +200,20 @@ struct Model {
         }
         for (int i = 0; i < s.dispatched && !status; i++) {
             if (s.w[i].done) continue;
+            if (proto == PICK && s.w[i].pc == 1) {
+                // the first wave's decision: heavy (front) or light (back),
+                // whatever its SIMD's count and the launch's rule say
+                for (int heavy = 0; heavy < 2 && !status; heavy++) {
+                    State t = s;
+                    t.w[i].first = heavy != 0;
+                    t.w[i].pc = 2;
+                    trace.push_back((char)('0' + i));
+                    explore(t, trace);
+                    trace.pop_back();
+                }
+                any = true;
+                continue;
+            }
             State t = s;
             if (!step(t, i)) continue;
             if (t.w[i].done) t.alive--;

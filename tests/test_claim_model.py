@@ -4,10 +4,10 @@ Every interleaving of the waves' atomic steps is explored, with in-order
 dispatch into a bounded number of resident slots:
 
 - the shipped hand-out (t1_pick_block: per-SIMD arrival counter, heavy
-  positions from the front for a SIMD's first wave unless three waves share
-  its SIMD when its bounded wait ends, light ones from the back for every
-  other wave) codes every block exactly once and never blocks, for every
-  resident count, including fewer slots than waves;
+  positions from the front for a SIMD's first wave unless it decides for a
+  light one after its bounded wait, light ones from the back for every other
+  wave) codes every block exactly once and never blocks, for every resident
+  count, including fewer slots than waves, and for either decision;
 - a persistent claim loop without waits does the same;
 - the exclusive-SIMD form, in which a later wave on a SIMD waits for the
   first wave's heavy/light publication, hangs in some interleavings (a first
