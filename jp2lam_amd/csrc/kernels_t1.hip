@@ -521,9 +521,11 @@ __device__ __forceinline__ uint32_t form_cup(const Tables& t, Col& st, uint64_t 
 // its table entry (qe | nmps << 16 | nlps << 22) with the MPS in bit 31, so
 // a decision needs one readlane; the next state is the table entry at the
 // NMPS or NLPS field XOR the MPS bit (entries 47..49, the LPS targets of the
-// switching states, carry bit 31 to flip it).  (Keeping the states in
-// an SGPR block as the decoder does measured no faster here: the state
-// write sits early enough in the renormalising path to hide its latency.)  Output bytes collect in
+// switching states, carry bit 31 to flip it).  The pre-formed heaviest
+// blocks keep the states in the decoder's SGPR block instead (mqe_group16s,
+// mqe_symbol_s): faster for a lone heavy wave, but for every block it
+// measured slower, as the two extra scalar instructions per decision then
+// compete with the waves sharing the SIMD.  Output bytes collect in
 // a 64-byte ring in one VGPR: byte b sits in lane (b + 1) & 63, so the byte
 // committed when pos moves on goes to lane pos & 63 with no range test (the
 // first commit, of no byte, lands in lane 0 as "byte -1", never stored).
