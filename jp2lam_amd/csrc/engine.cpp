@@ -847,8 +847,16 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         last_seg_offs_.resize(nb);
         for (int b = 0; b < nb; b++) last_seg_offs_[b] = jobs[b].out_off;
     }
-    J2L_HIP(hipStreamSynchronize(st));
-    t_metrics_ms = m_seg.empty() ? 0.0 : ev.ms(4, 5);
+    // The packet headers need only the selection: they are formed while the
+    // bodies come back, and the first image to write its bodies waits for
+    // the copies (once, for every image and thread).
+    std::once_flag landed;
+    auto bodies_landed = [&] { std::call_once(landed, [&] { J2L_HIP(hipStreamSynchronize(st)); }); };
+    struct LandGuard {  // an exception out of the host work still waits for the copies into the arena
+        hipStream_t s;
+        std::once_flag* f;
+        ~LandGuard() { std::call_once(*f, [this] { (void)hipStreamSynchronize(s); }); }
+    } land_guard{st, &landed};
 
     // ---- host Tier-2 + container --------------------------------------------------------
     auto th0 = std::chrono::steady_clock::now();
@@ -955,6 +963,7 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
         std::vector<uint8_t> head;
         write_codestream_head(p, hdrs.size() + bo, head);
         const size_t cs_len = head.size() + hdrs.size() + bo + 2;  // + EOC
+        bodies_landed();
         std::vector<uint8_t>& o = out.bytes;
         o.clear();
         o.reserve(cs_len + (p.format == 0 ? 128 : 0));
@@ -1007,6 +1016,8 @@ std::vector<EncOut> Engine::encode(const std::vector<EncItem>& items) {
                 outs[imgs[k].idx].message = errs[k];
             }
     }
+    bodies_landed();  // also when no image got that far; the debug copies and events are read later
+    t_metrics_ms = m_seg.empty() ? 0.0 : ev.ms(4, 5);
     t_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     return outs;
 }
@@ -1306,7 +1317,16 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     auto stage = [&](size_t off, const void* src, size_t bytes) {
         if (bytes) staged.push_back({off, H.add(bytes), bytes, src});
     };
-    for (auto& im : imgs) stage(o_pay + im.pay_off, im.ci.payload, im.ci.payload_len);
+    // A split decode uploads only the payload bytes its first part reads
+    // before that part's launch (the coarse resolutions' packets come first
+    // in the codestream); the tail goes up on the side stream while the
+    // first part runs.
+    size_t pay_head = 0;
+    if (split_res >= 0) {
+        for (int k = 0; k < nj; k++) pay_head = std::max(pay_head, (size_t)jobs[k].data_off + jobs[k].data_len);
+        pay_head = std::min(pay_head, imgs[0].ci.payload_len);
+    }
+    for (auto& im : imgs) stage(o_pay + im.pay_off, im.ci.payload, split_res >= 0 ? pay_head : im.ci.payload_len);
     if (nj) {
         rank_jobs(jobs.data(), nj, split_res >= 0 ? 15 : 1);
         stage(o_jobs, jobs.data(), sizeof(DecodeJob) * nj);
@@ -1317,6 +1337,8 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
     // a split decode stages its second job table after these (the first
     // uploads may still be reading the arena then)
     const size_t h_rest = H.add(sizeof(DecodeJob) * (size_t)(nj_all - nj));
+    const size_t pay_tail = split_res >= 0 ? imgs[0].ci.payload_len - pay_head : 0;
+    const size_t h_pay_tail = H.add(pay_tail);
     pin_.reserve(H.used);
     copy_staged(staged, D, pin_.base, st);
 
@@ -1359,7 +1381,11 @@ std::vector<DecOut> Engine::decode(const std::vector<DecItem>& items) {
         if (nr) {
             rank_jobs(rest.data(), nr, 1);
             std::memcpy(pin_.base + h_rest, rest.data(), sizeof(DecodeJob) * nr);
+            if (pay_tail) std::memcpy(pin_.base + h_pay_tail, im.ci.payload + pay_head, pay_tail);
             J2L_HIP(hipStreamWaitEvent(side_, ev_up_, 0));
+            if (pay_tail)
+                J2L_HIP(hipMemcpyAsync(D + o_pay + pay_head, pin_.base + h_pay_tail, pay_tail, hipMemcpyHostToDevice,
+                                       side_));
             J2L_HIP(hipMemcpyAsync(D + o_jobs + sizeof(DecodeJob) * nj, pin_.base + h_rest, sizeof(DecodeJob) * nr,
                                    hipMemcpyHostToDevice, side_));
             launch_t1_decode((const DecodeJob*)(D + o_jobs) + nj, nr, tables_, (const uint8_t*)(D + o_pay), side_);
