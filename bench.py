@@ -19,6 +19,17 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# HIP maps this process's streams onto GPU_MAX_HW_QUEUES hardware queues (4
+# by default), and two streams on one queue run one after the other.  The
+# run holds up to eight streams: torch's, the process-wide engine's and its
+# single-image decode side stream, the two serving slots and the three queue
+# engines.  With four queues the two serving slots shared one and ran in
+# turn: 64-page bursts 670-684 pages/s with four queues, 971-1,032 with
+# eight, same box.  Raised here, before anything starts HIP.
+HW_QUEUES = 8
+if int(os.environ.get("GPU_MAX_HW_QUEUES") or 0) < HW_QUEUES:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 HBM_PEAK_GBS = 8000.0
 
 
@@ -626,7 +637,8 @@ def main():
         "data": "synthetic (seeded document page: background 245+N(0,3), 12% dark glyph rectangles)",
         "config": {"workload": "plain per-image GPU path: 2550x3300 gray document page, q85 9/7 5 levels, "
                                "64x64 code-blocks, JP2; encode then decode, pixels resident in HBM",
-                   "width": W, "height": H, "quality": Q, "codestream_bytes": len(enc)},
+                   "width": W, "height": H, "quality": Q, "codestream_bytes": len(enc),
+                   "gpu_max_hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
         "roofline": roof,
         "cpu_baseline": cpu,
         "stage_ms": stage_ms,
