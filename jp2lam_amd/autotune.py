@@ -31,6 +31,7 @@ import argparse
 import ctypes as C
 import json
 import math
+import os
 import sys
 import time
 from pathlib import Path
@@ -41,11 +42,18 @@ PKG = Path(__file__).resolve().parent
 if str(PKG.parent) not in sys.path:
     sys.path.insert(0, str(PKG.parent))
 
+# The queue sweep runs up to three engines (a stream each, plus the
+# single-image decode's side stream): with HIP's default of four hardware
+# queues, streams share queues and run in turn, which would skew the sweep.
+# Raised before anything starts HIP, as bench.py does.
+if int(os.environ.get("GPU_MAX_HW_QUEUES") or 0) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 import jp2lam_amd as j  # noqa: E402
 from jp2lam_amd import _lib, synth  # noqa: E402
 
 # (name, t1 mode, hybrid split fraction)
-CONFIGS = [("wave", 1, 0.0), ("lanes", 2, 0.0)] + [(f"hybrid{f:.2f}", 4, f) for f in (0.55, 0.65, 0.75, 0.85)]
+CONFIGS = [("wave", 1, 0.0), ("lanes", 2, 0.0)] + [(f"hybrid{f:.2f}", 4, f) for f in (0.55, 0.6, 0.65, 0.7, 0.75, 0.85)]
 CODER_ID = {1: "kT1Wave", 2: "kT1Lanes", 4: "kT1Hybrid"}
 
 
